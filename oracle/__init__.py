@@ -1,0 +1,292 @@
+"""ctypes wrapper of the oracle (oracle/liboracle.so) and, when built, of the
+reference-compiled driver (oracle/_ref/libbhref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only as the checker / timed CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+REF_SO = os.path.join(_HERE, "_ref", "libbhref.so")
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_D = ctypes.c_double
+_I = ctypes.c_int
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _layout():
+    import sys
+
+    pkg = os.path.join(os.path.dirname(_HERE), "amt-bh-benchmark_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    from bhgpu import layout
+
+    return layout
+
+
+class OracleTree:
+    """A tree built by the oracle, in the reference's node layout."""
+
+    def __init__(self, lib, handle, n):
+        self._lib = lib
+        self._h = handle
+        self.n = n
+        nn, ng = _U64(), _U64()
+        dom = np.zeros(4)
+        lib.bho_tree_sizes(self._h, ctypes.byref(nn), ctypes.byref(ng), _ptr(dom))
+        self.n_nodes, self.n_groups, self.domain = nn.value, ng.value, dom
+
+    def export(self):
+        L = _layout()
+        n, m = self.n, self.n_nodes
+        out = dict(
+            keys=np.zeros(n, np.uint64),
+            perm=np.zeros(n, np.uint32),
+            particles=np.zeros(n, L.PARTICLE),
+            nodes=np.zeros(m, L.WALK_NODE),
+            center=np.zeros((m, 3)),
+            iab=np.zeros((m, 6)),
+            groups=np.zeros(self.n_groups, np.uint32),
+        )
+        self._lib.bho_tree_export(
+            self._h, _ptr(out["keys"]), _ptr(out["perm"]), _ptr(out["particles"]), _ptr(out["nodes"]),
+            _ptr(out["center"]), _ptr(out["iab"]), _ptr(out["groups"]),
+        )
+        out["domain"] = self.domain.copy()
+        return out
+
+    def _walk(self, fn, fp, target_size, threads):
+        acc = np.zeros((self.n, 3))
+        npp = np.zeros(self.n, np.uint32)
+        npc = np.zeros(self.n, np.uint32)
+        tot = np.zeros(2, np.uint64)
+        fn(self._h, _ptr(fp), target_size, threads, _ptr(acc), _ptr(npp), _ptr(npc), _ptr(tot))
+        return acc, npp, npc, tot
+
+    def grouped_pass(self, fp, target_size, threads=0):
+        """Accelerations (Morton order), per-particle P2P list length and
+        quadrupole-cell count, and their totals, from the shared group walk."""
+        return self._walk(self._lib.bho_grouped_pass, fp, target_size, threads)
+
+    def target_walk(self, fp, target_size, threads=0):
+        """The same from independent per-target walks from the root."""
+        return self._walk(self._lib.bho_target_walk, fp, target_size, threads)
+
+    def targets(self, target_size):
+        cap = self.n
+        out = np.zeros(cap, np.uint32)
+        k = self._lib.bho_targets(self._h, target_size, _ptr(out), cap)
+        return out[:k]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.bho_tree_free(self._h)
+            self._h = None
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        lib = ctypes.CDLL(path)
+        lib.bho_domain.argtypes = [_P, _U64, _P]
+        lib.bho_morton_keys.argtypes = [_P, _U64, _P, _P]
+        lib.bho_spread_bits3.argtypes = [_U32]
+        lib.bho_spread_bits3.restype = _U64
+        lib.bho_build.argtypes = [_P, _U64, _U32, _U32, _I, ctypes.POINTER(_P)]
+        lib.bho_tree_free.argtypes = [_P]
+        lib.bho_tree_sizes.argtypes = [_P, _P, _P, _P]
+        lib.bho_tree_export.argtypes = [_P] * 8
+        for fn in (lib.bho_grouped_pass, lib.bho_target_walk):
+            fn.argtypes = [_P, _P, _U32, _I, _P, _P, _P, _P]
+        lib.bho_targets.argtypes = [_P, _U32, _P, _U64]
+        lib.bho_targets.restype = _U64
+        lib.bho_step.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _D, _D, _I, _P]
+        lib.bho_baseline_sample.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _U32, _I, _P]
+        lib.bho_brute_force.argtypes = [_P, _U64, _D, _D, _I, _P]
+        lib.bho_brute_subset.argtypes = [_P, _U64, _P, _U64, _D, _D, _I, _P]
+        lib.bho_accel_from_source.argtypes = [_P, _D, _P, _P, _D, _I, _D, _P]
+        lib.bho_kinetic_energy.argtypes = [_P, _U64]
+        lib.bho_kinetic_energy.restype = _D
+        lib.bho_potential_energy.argtypes = [_P, _U64, _D, _I]
+        lib.bho_potential_energy.restype = _D
+        self.lib = lib
+
+    def domain(self, p):
+        out = np.zeros(4)
+        if self.lib.bho_domain(_ptr(p), len(p), _ptr(out)) != 0:
+            raise ValueError("empty input")
+        return out
+
+    def morton_keys(self, p, dom):
+        keys = np.zeros(len(p), np.uint64)
+        self.lib.bho_morton_keys(_ptr(p), len(p), _ptr(np.ascontiguousarray(dom, np.float64)), _ptr(keys))
+        return keys
+
+    def spread_bits3(self, v):
+        return self.lib.bho_spread_bits3(v)
+
+    def build(self, p, max_leaf, group_size, threads=0):
+        p = np.ascontiguousarray(p)
+        h = _P()
+        rc = self.lib.bho_build(_ptr(p), len(p), max_leaf, group_size, threads, ctypes.byref(h))
+        if rc != 0:
+            raise ValueError("oracle build rejected the input")
+        return OracleTree(self.lib, h, len(p))
+
+    def step(self, p, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, threads=0):
+        """step_particles (grouped branch): returns new particles in Morton order."""
+        out = np.array(p, copy=True)
+        timing = np.zeros(3)
+        rc = self.lib.bho_step(_ptr(out), len(out), _ptr(fp), max_leaf, group_size, target_size, kick_dt, drift_dt,
+                               threads, _ptr(timing))
+        if rc != 0:
+            raise ValueError("oracle step rejected the input")
+        return out, timing
+
+    def baseline_sample(self, p, fp, max_leaf, group_size, target_size, stride, threads=0):
+        out = np.zeros(3)
+        rc = self.lib.bho_baseline_sample(_ptr(p), len(p), _ptr(fp), max_leaf, group_size, target_size, stride, threads,
+                                          _ptr(out))
+        if rc != 0:
+            raise ValueError("baseline sample rejected the input")
+        return out
+
+    def brute_force(self, p, eps2, G=1.0, threads=0):
+        acc = np.zeros((len(p), 3))
+        self.lib.bho_brute_force(_ptr(p), len(p), eps2, G, threads, _ptr(acc))
+        return acc
+
+    def brute_subset(self, p, idx, eps2, G=1.0, threads=0):
+        idx = np.ascontiguousarray(idx, np.uint32)
+        acc = np.zeros((len(idx), 3))
+        self.lib.bho_brute_subset(_ptr(p), len(p), _ptr(idx), len(idx), eps2, G, threads, _ptr(acc))
+        return acc
+
+    def accel_from_source(self, tp, mass, com, iab, eps2, quad, G):
+        out = np.zeros(3)
+        self.lib.bho_accel_from_source(_ptr(np.asarray(tp, np.float64)), mass, _ptr(np.asarray(com, np.float64)),
+                                       _ptr(np.asarray(iab, np.float64)), eps2, int(quad), G, _ptr(out))
+        return out
+
+    def kinetic_energy(self, p):
+        return self.lib.bho_kinetic_energy(_ptr(p), len(p))
+
+    def potential_energy(self, p, G=1.0, threads=0):
+        return self.lib.bho_potential_energy(_ptr(p), len(p), G, threads)
+
+
+class RefTree:
+    def __init__(self, lib, h, n):
+        self._lib, self._h, self.n = lib, h, n
+        nn, ng = _U64(), _U64()
+        lib.ref_tree_sizes(h, ctypes.byref(nn), ctypes.byref(ng))
+        self.n_nodes, self.n_groups = nn.value, ng.value
+
+    def export(self):
+        L = _layout()
+        n, m = self.n, self.n_nodes
+        out = dict(
+            keys=np.zeros(n, np.uint64),
+            perm=np.zeros(n, np.uint32),
+            nodes=np.zeros(m, L.WALK_NODE),
+            center=np.zeros((m, 3)),
+            iab=np.zeros((m, 6)),
+            groups=np.zeros(self.n_groups, np.uint32),
+        )
+        self._lib.ref_tree_export(self._h, _ptr(out["keys"]), _ptr(out["perm"]), _ptr(out["nodes"]), _ptr(out["center"]),
+                                  _ptr(out["iab"]), _ptr(out["groups"]))
+        return out
+
+    def target_walk(self, theta, eps2, G, use_quadrupole, target_size):
+        acc = np.zeros((self.n, 3))
+        npp = np.zeros(self.n, np.uint32)
+        npc = np.zeros(self.n, np.uint32)
+        self._lib.ref_target_walk(self._h, theta, eps2, G, int(use_quadrupole), target_size, _ptr(acc), _ptr(npp), _ptr(npc))
+        return acc, npp, npc
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.ref_tree_free(self._h)
+            self._h = None
+
+
+class Reference:
+    """The reference's own functions, compiled from its sources (see
+    oracle/ref_driver.cpp). Only available where oracle/_ref was built."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = ctypes.CDLL(path)
+        lib.ref_domain.argtypes = [_P, _U64, _P]
+        lib.ref_morton_keys.argtypes = [_P, _U64, _P, _P]
+        lib.ref_spread_bits3.argtypes = [_U32]
+        lib.ref_spread_bits3.restype = _U64
+        lib.ref_build.argtypes = [_P, _U64, _U32, _U32, ctypes.POINTER(_P)]
+        lib.ref_tree_free.argtypes = [_P]
+        lib.ref_tree_sizes.argtypes = [_P, _P, _P]
+        lib.ref_tree_export.argtypes = [_P] * 7
+        lib.ref_target_walk.argtypes = [_P, _D, _D, _D, _I, _U32, _P, _P, _P]
+        lib.ref_brute_force.argtypes = [_P, _U64, _D, _D, _P]
+        lib.ref_accel_from_source.argtypes = [_P, _D, _P, _P, _D, _I, _D, _P]
+        lib.ref_kick_drift.argtypes = [_P, _U64, _P, _D, _D]
+        lib.ref_kinetic_energy.argtypes = [_P, _U64]
+        lib.ref_kinetic_energy.restype = _D
+        lib.ref_potential_energy.argtypes = [_P, _U64, _D]
+        lib.ref_potential_energy.restype = _D
+        self.lib = lib
+
+    def domain(self, p):
+        out = np.zeros(4)
+        self.lib.ref_domain(_ptr(p), len(p), _ptr(out))
+        return out
+
+    def morton_keys(self, p, dom):
+        keys = np.zeros(len(p), np.uint64)
+        self.lib.ref_morton_keys(_ptr(p), len(p), _ptr(np.ascontiguousarray(dom, np.float64)), _ptr(keys))
+        return keys
+
+    def build(self, p, max_leaf, group_size):
+        p = np.ascontiguousarray(p)
+        h = _P()
+        if self.lib.ref_build(_ptr(p), len(p), max_leaf, group_size, ctypes.byref(h)) != 0:
+            raise ValueError("empty input")
+        return RefTree(self.lib, h, len(p))
+
+    def brute_force(self, p, eps2, G=1.0):
+        acc = np.zeros((len(p), 3))
+        self.lib.ref_brute_force(_ptr(p), len(p), eps2, G, _ptr(acc))
+        return acc
+
+    def accel_from_source(self, tp, mass, com, iab, eps2, quad, G):
+        out = np.zeros(3)
+        self.lib.ref_accel_from_source(_ptr(np.asarray(tp, np.float64)), mass, _ptr(np.asarray(com, np.float64)),
+                                       _ptr(np.asarray(iab, np.float64)), eps2, int(quad), G, _ptr(out))
+        return out
+
+    def kick_drift(self, p, acc, kick_dt, drift_dt):
+        out = np.array(p, copy=True)
+        self.lib.ref_kick_drift(_ptr(out), len(out), _ptr(np.ascontiguousarray(acc, np.float64)), kick_dt, drift_dt)
+        return out
+
+    def kinetic_energy(self, p):
+        return self.lib.ref_kinetic_energy(_ptr(p), len(p))
+
+    def potential_energy(self, p, G=1.0):
+        return self.lib.ref_potential_energy(_ptr(p), len(p), G)
+
+
+def reference_available():
+    return os.path.exists(REF_SO)
