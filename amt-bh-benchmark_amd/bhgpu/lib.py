@@ -1,0 +1,208 @@
+"""ctypes binding of libbhgpu.so (include/bh_gpu.h).
+
+Loading fails loudly when the library has not been built: there is no
+fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .layout import FORCE_PARAMS, PARTICLE, WALK_NODE
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libbhgpu.so")
+
+BH_OK = 0
+STATUS = {
+    0: "BH_OK",
+    1: "BH_ERR_INVALID",
+    2: "BH_ERR_NONFINITE",
+    3: "BH_ERR_INCOMPLETE",
+    4: "BH_ERR_OOM",
+    5: "BH_ERR_HIP",
+    6: "BH_ERR_TOO_LARGE",
+    7: "BH_ERR_NO_TREE",
+}
+BH_FP32, BH_FP64 = 0, 1
+BH_LAYOUT_REFERENCE, BH_LAYOUT_BFS = 0, 1
+
+# Every entry point of include/bh_gpu.h with its ctypes signature.
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_D = ctypes.c_double
+_I = ctypes.c_int
+SIGNATURES = {
+    "bh_ctx_create": ([ctypes.POINTER(_P), _I], _I),
+    "bh_ctx_destroy": ([_P], None),
+    "bh_last_error": ([_P], ctypes.c_char_p),
+    "bh_set_precision": ([_P, _I], _I),
+    "bh_max_particles": ([], _U64),
+    "bh_step": ([_P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
+    "bh_step_device": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
+    "bh_state_upload": ([_P, _P, _U64], _I),
+    "bh_state_step": ([_P, _P, _U32, _U32, _U32, _D, _D, _P], _I),
+    "bh_state_download": ([_P, _P, _U64], _I),
+    "bh_state_device_ptr": ([_P], _P),
+    "bh_build_tree": ([_P, _P, _U64, _U32, _U32], _I),
+    "bh_tree_sizes": ([_P, _P, _P, _P], _I),
+    "bh_tree_export": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
+    "bh_accel": ([_P, _P, _U32, _P, _P, _P, _P], _I),
+    "bh_direct_sum": ([_P, _P, _U64, _P, _U64, _D, _D, _P], _I),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """The library handle; raises if libbhgpu.so is missing or incomplete."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built: run __graft_entry__.build() or make -C amt-bh-benchmark_amd")
+        lib = ctypes.CDLL(path)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+class BHError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class StepTiming(ctypes.Structure):
+    _fields_ = [("tree_build_ms", ctypes.c_double), ("traversal_ms", ctypes.c_double), ("exchange_ms", ctypes.c_double)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fp(fp):
+    if isinstance(fp, np.ndarray) and fp.dtype == FORCE_PARAMS:
+        return fp
+    raise TypeError("force params must come from bhgpu.force_params()")
+
+
+class Context:
+    """A bh_ctx on one GPU."""
+
+    def __init__(self, device=0):
+        self._lib = load()
+        h = _P()
+        rc = self._lib.bh_ctx_create(ctypes.byref(h), device)
+        if rc != BH_OK:
+            raise BHError(rc, f"bh_ctx_create(device={device}) failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bh_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != BH_OK:
+            raise BHError(rc, self._lib.bh_last_error(self._h).decode())
+
+    def set_precision(self, precision):
+        self._check(self._lib.bh_set_precision(self._h, precision))
+
+    def step(self, p, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
+        """step_particles: returns (particles in Morton order, StepTiming)."""
+        out = np.array(p, dtype=PARTICLE, copy=True)
+        t = StepTiming()
+        self._check(self._lib.bh_step(self._h, _ptr(out), len(out), _ptr(_fp(fp)), max_leaf, group_size, target_size,
+                                      kick_dt, drift_dt, ctypes.byref(t)))
+        return out, t
+
+    def step_device(self, d_in, d_out, n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
+        t = StepTiming()
+        self._check(self._lib.bh_step_device(self._h, d_in, d_out, n, _ptr(_fp(fp)), max_leaf, group_size, target_size,
+                                             kick_dt, drift_dt, ctypes.byref(t)))
+        return t
+
+    def state_upload(self, p):
+        p = np.ascontiguousarray(p, dtype=PARTICLE)
+        self._check(self._lib.bh_state_upload(self._h, _ptr(p), len(p)))
+        self._n = len(p)
+
+    def state_step(self, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
+        t = StepTiming()
+        self._check(self._lib.bh_state_step(self._h, _ptr(_fp(fp)), max_leaf, group_size, target_size, kick_dt, drift_dt,
+                                            ctypes.byref(t)))
+        return t
+
+    def state_download(self):
+        out = np.zeros(self._n, PARTICLE)
+        self._check(self._lib.bh_state_download(self._h, _ptr(out), len(out)))
+        return out
+
+    def state_device_ptr(self):
+        return self._lib.bh_state_device_ptr(self._h)
+
+    def build_tree(self, p, max_leaf, group_size):
+        p = np.ascontiguousarray(p, dtype=PARTICLE)
+        self._check(self._lib.bh_build_tree(self._h, _ptr(p), len(p), max_leaf, group_size))
+        self._n = len(p)
+
+    def tree_sizes(self):
+        nn, ng = _U64(), _U64()
+        dom = np.zeros(4)
+        self._check(self._lib.bh_tree_sizes(self._h, ctypes.byref(nn), ctypes.byref(ng), _ptr(dom)))
+        return nn.value, ng.value, dom
+
+    def export(self, layout=BH_LAYOUT_REFERENCE):
+        """The last built tree, as the oracle's export (reference node layout by default)."""
+        m, g, dom = self.tree_sizes()
+        n = self._n
+        out = dict(
+            keys=np.zeros(n, np.uint64),
+            perm=np.zeros(n, np.uint32),
+            particles=np.zeros(n, PARTICLE),
+            nodes=np.zeros(m, WALK_NODE),
+            center=np.zeros((m, 3)),
+            iab=np.zeros((m, 6)),
+            groups=np.zeros(g, np.uint32),
+        )
+        self._check(self._lib.bh_tree_export(self._h, layout, _ptr(out["keys"]), _ptr(out["perm"]), _ptr(out["particles"]),
+                                             _ptr(out["nodes"]), _ptr(out["center"]), _ptr(out["iab"]),
+                                             _ptr(out["groups"])))
+        out["domain"] = dom
+        return out
+
+    def accel(self, fp, target_size):
+        """run_grouped_pass on the last tree: (acc, n_pp, n_pc, totals), Morton order."""
+        n = self._n
+        acc = np.zeros((n, 3))
+        npp = np.zeros(n, np.uint32)
+        npc = np.zeros(n, np.uint32)
+        tot = np.zeros(2, np.uint64)
+        self._check(self._lib.bh_accel(self._h, _ptr(_fp(fp)), target_size, _ptr(acc), _ptr(npp), _ptr(npc), _ptr(tot)))
+        return acc, npp, npc, tot
+
+    def direct_sum(self, p, idx, eps2, G=1.0):
+        p = np.ascontiguousarray(p, dtype=PARTICLE)
+        if idx is None:
+            m = len(p)
+            acc = np.zeros((m, 3))
+            self._check(self._lib.bh_direct_sum(self._h, _ptr(p), len(p), None, 0, eps2, G, _ptr(acc)))
+        else:
+            idx = np.ascontiguousarray(idx, np.uint32)
+            acc = np.zeros((len(idx), 3))
+            self._check(self._lib.bh_direct_sum(self._h, _ptr(p), len(p), _ptr(idx), len(idx), eps2, G, _ptr(acc)))
+        return acc

@@ -1,0 +1,103 @@
+// Internal interfaces between the host orchestration (bh_api.cpp) and the
+// CDNA4 kernels (bh_sort.hip, bh_tree.hip, bh_walk.hip). Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bh_gpu.h"
+
+namespace bhk {
+
+constexpr int kWave = 64;
+constexpr int kDepthCap = 21;           // flat_tree.h:102 (21 octal digits)
+constexpr int kLevels = kDepthCap + 1;  // depths 0..21
+constexpr uint32_t kMaxLeafCap = 1024;  // halo of the tree-build tile
+constexpr uint32_t kSortTile = 4096;    // keys per onesweep workgroup
+constexpr uint32_t kTreeTile = 4096;    // particles per tree-build workgroup
+constexpr int kTreeCols = 24;           // per tile: 22 level counts, leaf count, pad
+
+// Flag/value word of the decoupled look-back (30-bit counts).
+constexpr uint32_t kLbAgg = 1u << 30;
+constexpr uint32_t kLbPre = 2u << 30;
+constexpr uint32_t kLbVal = (1u << 30) - 1;
+
+// Device view of one tree (level-order BFS numbering).
+struct TreeDev {
+  uint32_t* begin;       // [M] first particle (Morton position)
+  uint32_t* count;       // [M] particles
+  uint8_t* depth;        // [M]
+  uint8_t* leaf;         // [M]
+  uint32_t* cs;          // [M] child start: level-order index of the first child slot
+  uint32_t* first_child; // [M] (0 for leaves)
+  uint8_t* nchild;       // [M]
+  uint32_t* parent;      // [M] (UINT32_MAX for the root)
+  uint32_t* leaf_idx;    // [M] index into the leaf list (leaves only)
+  double4* ctr;          // [M] geometric centre xyz, half width
+  double* mass;          // [M]
+  double* com;           // [3M]
+  double* iab;           // [6M] xx,yy,zz,xy,xz,yz about the com
+  double4* mac;          // [M] com xyz, (2*hw)^2
+  uint2* link;           // [M] leaf: (particle_begin, count | 1<<31); interior: (first_child, nchild)
+  float4* q32;           // [2M] {mass, Qxx, Qyy, Qzz}, {Qxy, Qxz, Qyz, 0}
+  double* q64;           // [8M] mass, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, 0
+  uint32_t* leaf_node;   // [leaves] node of each leaf, Morton order
+  uint32_t* leaf_first;  // [leaves] first particle of each leaf
+  uint32_t* level_off;   // [kLevels + 2] level start offsets, then total, total
+};
+
+struct WalkArgs {
+  const double4* mac;
+  const uint2* link;
+  const float4* q32;
+  const double* q64;
+  const double4* ctr;
+  const uint32_t* begin;
+  const uint32_t* count;
+  const double4* posm;     // x, y, z, m in Morton order
+  const uint32_t* targets; // target nodes, Morton order
+  uint32_t n_targets;
+  bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
+  double* acc;             // [3n] G * acceleration (nullable)
+  uint32_t* n_pp;          // [n] (nullable)
+  uint32_t* n_pc;          // [n] (nullable)
+  unsigned long long* totals;  // [2]
+  double theta2;
+  double eps2;
+  double G;
+  double kick_dt;
+  double drift_dt;
+  int use_quad;
+};
+
+// ---- bh_sort.hip
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+hipError_t launch_bounds(const bh_particle* p, uint32_t n, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s);
+hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const double* lohi, uint64_t* keys, uint32_t* hist,
+                            hipStream_t s);
+hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b, uint32_t n,
+                             uint32_t* hist, uint32_t* digit_base, uint32_t* lookback, uint32_t* tile_counters,
+                             hipStream_t s);
+hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
+                         hipStream_t s);
+uint64_t radix_lookback_words(uint32_t n);
+
+// ---- bh_tree.hip
+hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, uint32_t max_leaf, uint8_t* L, uint8_t* H,
+                             uint32_t* tile_counts, hipStream_t s);
+hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_t* tile_prefix, uint32_t* totals,
+                            hipStream_t s);
+hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix,
+                            const uint32_t* totals, TreeDev t, hipStream_t s);
+hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const double* lohi,
+                              const uint32_t* totals, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s);
+hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, uint32_t tsm, uint32_t* mark, uint32_t* tile_counts,
+                          uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets, hipStream_t s);
+
+// ---- bh_walk.hip
+hipError_t launch_walk(const WalkArgs& a, int fp64, hipStream_t s);
+hipError_t launch_direct(const double4* src, uint32_t n, const uint32_t* tidx, uint32_t m, double eps2, double G,
+                         double* partial, uint32_t splits, double* acc, hipStream_t s);
+uint32_t direct_splits(uint32_t n, uint32_t m);
+
+}  // namespace bhk

@@ -1,0 +1,389 @@
+// Bounding box, Morton keys and the stable (key, index) sort on CDNA4.
+//
+// Reference semantics:
+//   particle_bounds          flat_build_ttg.h:93-113
+//   domain_from_bounds       domain.h:84-91
+//   morton_key/spread_bits3  flat_tree.h:104-125
+//   sort order               flat_build_ttg.h:62-66 (ascending key, ties by input index)
+//   gather to Morton order   flat_build_ttg.h:180-188
+//
+// This file is compiled with -ffp-contract=off: the domain and key
+// arithmetic must round exactly as the reference's FP64 code does.
+//
+// The sort is an LSD radix sort over the 63-bit keys with the input index as
+// payload, 8 passes of 8-bit digits. LSD radix sorting is stable, so the
+// result is exactly the (key, index) order. Each pass is a single "onesweep"
+// kernel: a workgroup ranks a 4096-key tile in LDS with wavefront ballots,
+// obtains its global digit offsets by decoupled look-back over the preceding
+// tiles, and writes the tile out digit-sorted so stores coalesce.
+#include "bh_internal.h"
+
+namespace bhk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;  // kSortTile / kBlock
+
+__device__ inline uint64_t lanemask_lt() {
+  const int lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// domain.h:84-91 from the reduced bounds lohi = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z}.
+__device__ inline void domain_from_lohi(const double* lohi, double c[3], double* hw) {
+  c[0] = 0.5 * (lohi[0] + lohi[3]);
+  c[1] = 0.5 * (lohi[1] + lohi[4]);
+  c[2] = 0.5 * (lohi[2] + lohi[5]);
+  const double ex = lohi[3] - lohi[0], ey = lohi[4] - lohi[1], ez = lohi[5] - lohi[2];
+  // std::max({ex, ey, ez}): the first largest; equal values are identical
+  double m = ex;
+  if (m < ey) m = ey;
+  if (m < ez) m = ez;
+  double h = 0.5 * m;
+  if (!(h > 0.0)) h = 1.0;
+  h *= 1.0 + 1e-9;
+  *hw = h;
+}
+
+__device__ inline uint64_t spread_bits3(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+__device__ inline uint32_t quantize(double v, double lo, double scale) {
+  const double cells = 2097152.0;
+  double t = floor((v - lo) * scale);
+  t = t < 0.0 ? 0.0 : (cells - 1.0 < t ? cells - 1.0 : t);  // std::clamp(t, 0, cells - 1)
+  return static_cast<uint32_t>(t);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) p[i] = v;
+}
+
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Per-block min/max of the positions; flags any non-finite position,
+// velocity or mass.
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const bh_particle* __restrict__ p, uint32_t n,
+                                                           double* __restrict__ partial, uint32_t* __restrict__ nonfinite) {
+  const double inf = __builtin_huge_val();
+  double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+  bool bad = false;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const bh_particle& q = p[i];
+    const double x = q.pos[0], y = q.pos[1], z = q.pos[2];
+    bad |= !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(q.mass) && isfinite(q.vel[0]) && isfinite(q.vel[1]) &&
+             isfinite(q.vel[2]));
+    lo[0] = fmin(lo[0], x);
+    lo[1] = fmin(lo[1], y);
+    lo[2] = fmin(lo[2], z);
+    hi[0] = fmax(hi[0], x);
+    hi[1] = fmax(hi[1], y);
+    hi[2] = fmax(hi[2], z);
+  }
+  __shared__ double s[4][6];
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  double r[6];
+  for (int a = 0; a < 3; ++a) {
+    r[a] = wave_min(lo[a]);
+    r[3 + a] = wave_max(hi[a]);
+  }
+  if (lane == 0)
+    for (int a = 0; a < 6; ++a) s[w][a] = r[a];
+  if (__ballot(bad) != 0 && lane == 0) atomicOr(nonfinite, 1u);
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    double v = s[0][a];
+    for (int k = 1; k < 4; ++k) v = a < 3 ? fmin(v, s[k][a]) : fmax(v, s[k][a]);
+    partial[blockIdx.x * 6 + a] = v;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const double* __restrict__ partial, uint32_t blocks,
+                                                         double* __restrict__ lohi) {
+  const double inf = __builtin_huge_val();
+  __shared__ double s[4][6];
+  double r[6] = {inf, inf, inf, -inf, -inf, -inf};
+  for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
+    for (int a = 0; a < 3; ++a) {
+      r[a] = fmin(r[a], partial[b * 6 + a]);
+      r[3 + a] = fmax(r[3 + a], partial[b * 6 + 3 + a]);
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  for (int a = 0; a < 3; ++a) {
+    r[a] = wave_min(r[a]);
+    r[3 + a] = wave_max(r[3 + a]);
+  }
+  if (lane == 0)
+    for (int a = 0; a < 6; ++a) s[w][a] = r[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    double v = s[0][a];
+    for (int k = 1; k < 4; ++k) v = a < 3 ? fmin(v, s[k][a]) : fmax(v, s[k][a]);
+    lohi[a] = v;
+  }
+}
+
+// Morton key per particle (input order) and the 8 digit histograms of all
+// keys, for the onesweep passes' global digit offsets.
+__global__ __launch_bounds__(kBlock) void k_keys_hist(const bh_particle* __restrict__ p, uint32_t n,
+                                                      const double* __restrict__ lohi, uint64_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[8 * 256];
+  for (int i = threadIdx.x; i < 8 * 256; i += kBlock) s_h[i] = 0;
+  double c[3], hw;
+  domain_from_lohi(lohi, c, &hw);
+  const double scale = 2097152.0 / (2.0 * hw);
+  const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const bh_particle& q = p[i];
+    const uint64_t key = spread_bits3(quantize(q.pos[0], lx, scale)) | (spread_bits3(quantize(q.pos[1], ly, scale)) << 1) |
+                         (spread_bits3(quantize(q.pos[2], lz, scale)) << 2);
+    keys[i] = key;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) atomicAdd(&s_h[b * 256 + ((key >> (8 * b)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 8 * 256; i += kBlock)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// Exclusive block scan of one value per thread (256 threads).
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum, uint32_t* total) {
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t u = __shfl_up(x, o);
+    if (lane >= o) x += u;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int k = 0; k < kBlock / 64; ++k) {
+    if (k < w) off += s_wsum[k];
+    tot += s_wsum[k];
+  }
+  if (total) *total = tot;
+  __syncthreads();
+  return off + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hist_scan(const uint32_t* __restrict__ hist, uint32_t* __restrict__ dbase) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t pass = blockIdx.x;
+  const uint32_t v = hist[pass * 256 + threadIdx.x];
+  dbase[pass * 256 + threadIdx.x] = block_excl_scan(v, s_w, nullptr);
+}
+
+template <bool kFirst>
+__global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                     uint32_t shift, const uint32_t* __restrict__ dbase,
+                                                     uint32_t* __restrict__ lookback, uint32_t* __restrict__ tile_counter) {
+  __shared__ uint32_t s_hist[4][256];   // per-wave digit counters, then per-wave exclusive bases
+  __shared__ uint32_t s_lexcl[256];     // tile-local exclusive digit offsets
+  __shared__ uint32_t s_adj[256];       // global position - local position, per digit
+  __shared__ uint64_t s_k[kSortTile];
+  __shared__ uint32_t s_v[kSortTile];
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_tile;
+
+  const int tid = threadIdx.x, w = tid >> 6, lane = __lane_id();
+  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = tid; i < 4 * 256; i += kBlock) (&s_hist[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * kSortTile;
+
+  uint64_t k[kItems];
+  uint32_t v[kItems];
+  uint32_t r[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
+    const bool ok = idx < n;
+    k[j] = ok ? kin[idx] : ~0ull;
+    v[j] = kFirst ? idx : (ok ? vin[idx] : 0u);
+  }
+  const uint64_t lt = lanemask_lt();
+  // Stable in-tile ranking: items in (wave, item, lane) order follow the
+  // input order; lanes with an equal digit are found with 8 ballots.
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
+    const bool ok = idx < n;
+    const uint32_t d = static_cast<uint32_t>(k[j] >> shift) & 255u;
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t below = __popcll(peers & lt);
+    const uint32_t cnt = s_hist[w][d];
+    r[j] = cnt + below;
+    __builtin_amdgcn_wave_barrier();
+    if (ok && below == 0) s_hist[w][d] = cnt + __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // per digit: totals over waves and per-wave exclusive bases
+  const uint32_t t = tid;
+  uint32_t total = 0;
+  {
+    uint32_t c0 = s_hist[0][t], c1 = s_hist[1][t], c2 = s_hist[2][t], c3 = s_hist[3][t];
+    s_hist[0][t] = 0;
+    s_hist[1][t] = c0;
+    s_hist[2][t] = c0 + c1;
+    s_hist[3][t] = c0 + c1 + c2;
+    total = c0 + c1 + c2 + c3;
+  }
+  uint32_t* my = &lookback[uint64_t(tile) * 256 + t];
+  __hip_atomic_store(my, (tile == 0 ? kLbPre : kLbAgg) | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t lex = block_excl_scan(total, s_w, nullptr);
+  s_lexcl[t] = lex;
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int64_t j = static_cast<int64_t>(tile) - 1;
+    for (;;) {
+      const uint32_t val = __hip_atomic_load(&lookback[uint64_t(j) * 256 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t f = val & ~kLbVal;
+      if (f == 0) continue;
+      excl += val & kLbVal;
+      if (f == kLbPre) break;
+      --j;
+    }
+    __hip_atomic_store(my, kLbPre | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_adj[t] = dbase[t] + excl - lex;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
+    if (idx < n) {
+      const uint32_t d = static_cast<uint32_t>(k[j] >> shift) & 255u;
+      const uint32_t pos = s_lexcl[d] + s_hist[w][d] + r[j];
+      s_k[pos] = k[j];
+      s_v[pos] = v[j];
+    }
+  }
+  __syncthreads();
+  const uint32_t tn = n - base < kSortTile ? n - base : kSortTile;
+  for (uint32_t e = tid; e < tn; e += kBlock) {
+    const uint64_t key = s_k[e];
+    const uint32_t d = static_cast<uint32_t>(key >> shift) & 255u;
+    const uint32_t g = s_adj[d] + e;
+    kout[g] = key;
+    vout[g] = s_v[e];
+  }
+}
+
+// Permutes the particles into Morton order and writes the FP64 walk copy
+// (x, y, z, m), as sort_particles' final loop does (flat_build_ttg.h:180-188).
+__global__ __launch_bounds__(kBlock) void k_gather(const bh_particle* __restrict__ in, bh_particle* __restrict__ out,
+                                                   const uint32_t* __restrict__ perm, double4* __restrict__ posm, uint32_t n) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t j = perm[k];
+  const uint4* src = reinterpret_cast<const uint4*>(in + j);
+  uint4* dst = reinterpret_cast<uint4*>(out + k);
+  const uint4 a = src[0], b = src[1], c = src[2], d = src[3];
+  dst[0] = a;
+  dst[1] = b;
+  dst[2] = c;
+  dst[3] = d;
+  double4 q;
+  q.x = __hiloint2double(b.y, b.x);
+  q.y = __hiloint2double(b.w, b.z);
+  q.z = __hiloint2double(c.y, c.x);
+  q.w = __hiloint2double(a.w, a.z);
+  posm[k] = q;
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  return static_cast<uint32_t>(g < cap ? g : cap);
+}
+
+}  // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, p, value, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_bounds(const bh_particle* p, uint32_t n, double* partial, uint32_t* nonfinite, double* lohi,
+                         hipStream_t s) {
+  const uint32_t g = grid_for(n, 1024);
+  hipLaunchKernelGGL(k_bounds_partial, dim3(g), dim3(kBlock), 0, s, p, n, partial, nonfinite);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, s, partial, g, lohi);
+  return hipGetLastError();
+}
+
+hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const double* lohi, uint64_t* keys, uint32_t* hist,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_keys_hist, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, p, n, lohi, keys, hist);
+  return hipGetLastError();
+}
+
+uint64_t radix_lookback_words(uint32_t n) {
+  const uint64_t tiles = (uint64_t(n) + kSortTile - 1) / kSortTile;
+  return 8 * tiles * 256 + 8;  // 8 passes of tiles x 256 words, then 8 tile counters
+}
+
+hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b, uint32_t n,
+                             uint32_t* hist, uint32_t* digit_base, uint32_t* lookback, uint32_t* tile_counters,
+                             hipStream_t s) {
+  const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+  hipLaunchKernelGGL(k_hist_scan, dim3(8), dim3(kBlock), 0, s, hist, digit_base);
+  uint64_t* kin = keys_a;
+  uint32_t* vin = vals_a;
+  uint64_t* ko = keys_b;
+  uint32_t* vo = vals_b;
+  for (int pass = 0; pass < 8; ++pass) {
+    uint32_t* lb = lookback + uint64_t(pass) * tiles * 256;
+    if (pass == 0)
+      hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 8u * pass,
+                         digit_base + pass * 256, lb, tile_counters + pass);
+    else
+      hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 8u * pass,
+                         digit_base + pass * 256, lb, tile_counters + pass);
+    uint64_t* tk = kin;
+    kin = ko;
+    ko = tk;
+    uint32_t* tv = vin;
+    vin = vo;
+    vo = tv;
+  }
+  // 8 passes: the sorted keys and indices end in (keys_a, vals_a)
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, out, perm, posm, n);
+  return hipGetLastError();
+}
+
+}  // namespace bhk

@@ -1,0 +1,443 @@
+// Octree topology, multipoles and walk targets from the Morton-sorted keys.
+//
+// Reference semantics (whole key range owned, one rank):
+//   LeafRule                 flat_tree.h:144-156   leaf iff count <= max_leaf || depth >= 21
+//   split_octants/split_node flat_tree.h:167-225   children = non-empty octants, ascending, consecutive
+//   child_center             flat_tree.h:182-185   c +- hw/2 per octant bit, hw/2
+//   finalize_node            flat_tree.h:257-274   raw moments about the cell centre, parallel-axis shift
+//   group roots              flat_tree.h:190-192
+//   targets                  grouped_ttg.h:253     topmost nodes with leaf || count <= target_size
+//
+// Construction without a level loop. For sorted keys K and particle i:
+//   L_i = 1 + (octal digits K[i-1] and K[i] share)      (L_0 = 0, 22 if equal)
+//         i is the first particle of its cell at every depth d >= L_i;
+//   P_u = digits shared by K[u] and K[u+max_leaf]       (-1 if u+max_leaf >= n)
+//         the depth-d cell of u holds > max_leaf particles iff d <= P_u;
+//   W_i = max P_u over u in [i-max_leaf, i]
+//         the depth-d cell containing i holds > max_leaf particles iff d <= W_i
+//         (a cell of >= max_leaf+1 particles containing i contains a window
+//          [u, u+max_leaf] with u <= i <= u+max_leaf, and conversely);
+//   H_i = min(W_i, 20) + 1 (0 if n <= max_leaf) = depth of the leaf holding i,
+//         since a cell is a node iff its parent is split (> max_leaf, depth < 21).
+// So particle i begins exactly the nodes (d, i) for d in [L_i, H_i]. Nodes are
+// numbered level by level (depth, then first particle): the level-order BFS
+// the reference's build_subtree produces for a tree of <= 4096 particles.
+// A node's children are the level d+1 nodes that begin inside it; with
+// cs(n) = number of level-(d+1) begins before begin(n), n's children are
+// [cs(n), cs(next node of level d)).
+//
+// Compiled with -ffp-contract=off: centres and multipoles round exactly as
+// the reference's FP64 code does.
+#include "bh_internal.h"
+
+namespace bhk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+
+__device__ inline uint64_t lanemask_lt() {
+  const int lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Number of leading octal digits (of 21) two 63-bit keys share.
+__device__ inline int common_digits(uint64_t a, uint64_t b) {
+  const uint64_t x = a ^ b;
+  return x == 0 ? kDepthCap : (__clzll(static_cast<long long>(x)) - 1) / 3;
+}
+
+__device__ inline void domain_from_lohi(const double* lohi, double c[3], double* hw) {
+  c[0] = 0.5 * (lohi[0] + lohi[3]);
+  c[1] = 0.5 * (lohi[1] + lohi[4]);
+  c[2] = 0.5 * (lohi[2] + lohi[5]);
+  const double ex = lohi[3] - lohi[0], ey = lohi[4] - lohi[1], ez = lohi[5] - lohi[2];
+  double m = ex;
+  if (m < ey) m = ey;
+  if (m < ez) m = ez;
+  double h = 0.5 * m;
+  if (!(h > 0.0)) h = 1.0;
+  h *= 1.0 + 1e-9;
+  *hw = h;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restrict__ keys, uint32_t n, uint32_t ml,
+                                                       uint8_t* __restrict__ L, uint8_t* __restrict__ H,
+                                                       uint32_t* __restrict__ tile_counts) {
+  __shared__ uint64_t s_k[kTreeTile + 2 * kMaxLeafCap + 1];
+  __shared__ int8_t s_p[kTreeTile + kMaxLeafCap];
+  __shared__ uint32_t s_cnt[kTreeCols];
+  const uint32_t tile = blockIdx.x, base = tile * kTreeTile;
+  const int64_t g0 = static_cast<int64_t>(base) - ml - 1;  // global index of s_k[0]
+  const uint32_t wk = kTreeTile + 2 * ml + 1;
+  for (uint32_t e = threadIdx.x; e < wk; e += kBlock) {
+    const int64_t g = g0 + e;
+    s_k[e] = (g >= 0 && g < n) ? keys[g] : 0ull;
+  }
+  if (threadIdx.x < kTreeCols) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  // P_u for u in [base - ml, base + tile)
+  for (uint32_t e = threadIdx.x; e < kTreeTile + ml; e += kBlock) {
+    const int64_t u = static_cast<int64_t>(base) - ml + e;
+    int8_t pv = -1;
+    if (u >= 0 && u + ml < n) pv = static_cast<int8_t>(common_digits(s_k[u - g0], s_k[u + ml - g0]));
+    s_p[e] = pv;
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < kTreeTile; e += kBlock) {
+    const uint32_t i = base + e;
+    if (i >= n) break;
+    const int li = (i == 0) ? 0 : common_digits(s_k[i - 1 - g0], s_k[i - g0]) + 1;
+    int wmax = -1;
+    for (uint32_t u = 0; u <= ml; ++u) wmax = max(wmax, static_cast<int>(s_p[e + u]));
+    const int hi = (n > ml) ? min(wmax, kDepthCap - 1) + 1 : 0;
+    L[i] = static_cast<uint8_t>(li);
+    H[i] = static_cast<uint8_t>(hi);
+    for (int d = li; d <= hi; ++d) atomicAdd(&s_cnt[d], 1u);
+    if (li <= hi) atomicAdd(&s_cnt[kLevels], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kTreeCols) tile_counts[uint64_t(tile) * kTreeCols + threadIdx.x] = s_cnt[threadIdx.x];
+}
+
+// Column-wise exclusive scan over tiles: one block per column.
+__global__ __launch_bounds__(1024) void k_col_scan(const uint32_t* __restrict__ in, uint32_t tiles, uint32_t stride,
+                                                   uint32_t* __restrict__ prefix, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_w[16];
+  const uint32_t col = blockIdx.x;
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < tiles; b += 1024) {
+    const uint32_t t = b + threadIdx.x;
+    const uint32_t v = t < tiles ? in[uint64_t(t) * stride + col] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(x, o);
+      if (lane >= o) x += u;
+    }
+    if (lane == 63) s_w[w] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+      if (k < w) off += s_w[k];
+      tot += s_w[k];
+    }
+    if (t < tiles) prefix[uint64_t(t) * stride + col] = carry + off + x - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) totals[col] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tree_emit(const uint8_t* __restrict__ L, const uint8_t* __restrict__ H, uint32_t n,
+                                                      const uint32_t* __restrict__ tile_prefix,
+                                                      const uint32_t* __restrict__ totals, TreeDev t) {
+  __shared__ uint32_t s_loff[kLevels + 2];
+  __shared__ uint32_t s_wc[4][kTreeCols];
+  const uint32_t tile = blockIdx.x, base = tile * kTreeTile;
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  if (threadIdx.x == 0) {
+    uint32_t off = 0;
+    for (int d = 0; d < kLevels; ++d) {
+      s_loff[d] = off;
+      off += totals[d];
+    }
+    s_loff[kLevels] = off;
+    s_loff[kLevels + 1] = off;
+    if (tile == 0)
+      for (int d = 0; d < kLevels + 2; ++d) t.level_off[d] = s_loff[d];
+  }
+  // phase 1: per-wave counts of level begins and leaf begins
+  uint32_t c[kLevels + 1];
+#pragma unroll
+  for (int d = 0; d <= kLevels; ++d) c[d] = 0;
+#pragma unroll 1
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t idx = base + w * (kTreeTile / 4) + j * 64 + lane;
+    const bool ok = idx < n;
+    const int lj = ok ? L[idx] : 255, hj = ok ? H[idx] : 0;
+#pragma unroll
+    for (int d = 0; d < kLevels; ++d) c[d] += __popcll(__ballot(lj <= d && d <= hj));
+    c[kLevels] += __popcll(__ballot(lj <= hj));
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d <= kLevels; ++d) s_wc[w][d] = c[d];
+  }
+  __syncthreads();
+  if (threadIdx.x <= kLevels) {
+    uint32_t a = 0;
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t v = s_wc[k][threadIdx.x];
+      s_wc[k][threadIdx.x] = a;
+      a += v;
+    }
+  }
+  __syncthreads();
+  // phase 2: global ranks
+  uint32_t run[kLevels + 1];
+#pragma unroll
+  for (int d = 0; d < kLevels; ++d) run[d] = s_loff[d] + tile_prefix[uint64_t(tile) * kTreeCols + d] + s_wc[w][d];
+  run[kLevels] = tile_prefix[uint64_t(tile) * kTreeCols + kLevels] + s_wc[w][kLevels];
+  const uint32_t total_nodes = s_loff[kLevels];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll 1
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t idx = base + w * (kTreeTile / 4) + j * 64 + lane;
+    const bool ok = idx < n;
+    const int lj = ok ? L[idx] : 255, hj = ok ? H[idx] : 0;
+    const uint64_t lm = __ballot(lj <= hj);
+    const uint32_t lr = run[kLevels] + __popcll(lm & lt);
+    run[kLevels] += __popcll(lm);
+    uint32_t prev = 0;
+#pragma unroll
+    for (int d = 0; d <= kLevels; ++d) {
+      uint32_t rank;
+      if (d < kLevels) {
+        const uint64_t m = __ballot(lj <= d && d <= hj);
+        rank = run[d] + __popcll(m & lt);
+        run[d] += __popcll(m);
+      } else {
+        rank = total_nodes;
+      }
+      if (d >= 1 && lj <= d - 1 && d - 1 <= hj) {
+        const uint32_t node = prev;
+        t.begin[node] = idx;
+        t.depth[node] = static_cast<uint8_t>(d - 1);
+        t.cs[node] = rank;
+        const bool leaf = (d - 1 == hj);
+        t.leaf[node] = leaf ? 1 : 0;
+        if (leaf) {
+          t.leaf_node[lr] = node;
+          t.leaf_first[lr] = idx;
+          t.leaf_idx[node] = lr;
+        }
+      }
+      prev = rank;
+    }
+  }
+}
+
+// Children, parents, leaf counts and cell geometry of every node.
+__global__ __launch_bounds__(kBlock) void k_tree_links(const uint64_t* __restrict__ keys, uint32_t n,
+                                                       const double* __restrict__ lohi, uint32_t n_nodes, uint32_t n_leaves,
+                                                       TreeDev t) {
+  double c0[3], hw0;
+  domain_from_lohi(lohi, c0, &hw0);
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
+    const int d = t.depth[i];
+    const uint32_t lend = t.level_off[d + 1];
+    const uint32_t next = (i + 1 < lend) ? t.cs[i + 1] : t.level_off[d + 2];
+    const uint32_t cs = t.cs[i];
+    if (i == 0) t.parent[0] = 0xffffffffu;
+    if (t.leaf[i]) {
+      t.first_child[i] = 0;
+      t.nchild[i] = 0;
+      const uint32_t lr = t.leaf_idx[i];
+      const uint32_t end = (lr + 1 < n_leaves) ? t.leaf_first[lr + 1] : n;
+      t.count[i] = end - t.leaf_first[lr];
+    } else {
+      const uint32_t k = next - cs;
+      t.first_child[i] = cs;
+      t.nchild[i] = static_cast<uint8_t>(k);
+      for (uint32_t c = 0; c < k; ++c) t.parent[cs + c] = i;
+    }
+    // child_center applied digit by digit from the root (flat_tree.h:182-185)
+    const uint64_t key = keys[t.begin[i]];
+    double cx = c0[0], cy = c0[1], cz = c0[2], h = hw0;
+    for (int e = 1; e <= d; ++e) {
+      const unsigned oct = static_cast<unsigned>((key >> (63 - 3 * e)) & 7u);
+      const double q = 0.5 * h;
+      cx = cx + ((oct & 1u) ? q : -q);
+      cy = cy + ((oct & 2u) ? q : -q);
+      cz = cz + ((oct & 4u) ? q : -q);
+      h = 0.5 * h;
+    }
+    t.ctr[i] = make_double4(cx, cy, cz, h);
+  }
+}
+
+// finalize_node for every node of one level (flat_tree.h:257-274), children
+// already final. Also derives the walk's node records.
+__global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ posm, int level, TreeDev t) {
+  const uint32_t lo = t.level_off[level], hi = t.level_off[level + 1];
+  for (uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x; i < hi; i += gridDim.x * kBlock) {
+    const double4 cc = t.ctr[i];
+    const double ref[3] = {cc.x, cc.y, cc.z};
+    double rm = 0.0, m1[3] = {0.0, 0.0, 0.0}, m2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    uint32_t cnt;
+    const bool leaf = t.leaf[i] != 0;
+    if (leaf) {
+      const uint32_t b = t.begin[i];
+      cnt = t.count[i];
+      for (uint32_t p = b; p < b + cnt; ++p) {
+        const double4 q = posm[p];
+        const double mass = q.w;
+        const double d[3] = {q.x - ref[0], q.y - ref[1], q.z - ref[2]};
+        rm += mass;
+        for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
+        m2[0] += mass * d[0] * d[0];
+        m2[1] += mass * d[1] * d[1];
+        m2[2] += mass * d[2] * d[2];
+        m2[3] += mass * d[0] * d[1];
+        m2[4] += mass * d[0] * d[2];
+        m2[5] += mass * d[1] * d[2];
+      }
+    } else {
+      const uint32_t fc = t.first_child[i], k = t.nchild[i];
+      cnt = 0;
+      for (uint32_t c = fc; c < fc + k; ++c) {
+        const double mass = t.mass[c];
+        const double d[3] = {t.com[3 * c + 0] - ref[0], t.com[3 * c + 1] - ref[1], t.com[3 * c + 2] - ref[2]};
+        rm += mass;
+        for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
+        const double* ib = &t.iab[6 * c];
+        m2[0] += ib[0] + mass * d[0] * d[0];
+        m2[1] += ib[1] + mass * d[1] * d[1];
+        m2[2] += ib[2] + mass * d[2] * d[2];
+        m2[3] += ib[3] + mass * d[0] * d[1];
+        m2[4] += ib[4] + mass * d[0] * d[2];
+        m2[5] += ib[5] + mass * d[1] * d[2];
+        cnt += t.count[c];
+      }
+      t.count[i] = cnt;
+    }
+    // finalize_raw_moment (multipole.h:76-87)
+    const double inv = 1.0 / rm;
+    const double dl[3] = {m1[0] * inv, m1[1] * inv, m1[2] * inv};
+    const double com[3] = {ref[0] + dl[0], ref[1] + dl[1], ref[2] + dl[2]};
+    double ib[6];
+    ib[0] = m2[0] - rm * dl[0] * dl[0];
+    ib[1] = m2[1] - rm * dl[1] * dl[1];
+    ib[2] = m2[2] - rm * dl[2] * dl[2];
+    ib[3] = m2[3] - rm * dl[0] * dl[1];
+    ib[4] = m2[4] - rm * dl[0] * dl[2];
+    ib[5] = m2[5] - rm * dl[1] * dl[2];
+    t.mass[i] = rm;
+    for (int a = 0; a < 3; ++a) t.com[3 * i + a] = com[a];
+    for (int a = 0; a < 6; ++a) t.iab[6 * i + a] = ib[a];
+    // walk records: MAC data, links, traceless quadrupole (multipole.h:185-188)
+    const double edge = 2.0 * cc.w;
+    t.mac[i] = make_double4(com[0], com[1], com[2], edge * edge);
+    t.link[i] = leaf ? make_uint2(t.begin[i], cnt | 0x80000000u) : make_uint2(t.first_child[i], t.nchild[i]);
+    const double tr = ib[0] + ib[1] + ib[2];
+    const double q[6] = {3.0 * ib[0] - tr, 3.0 * ib[1] - tr, 3.0 * ib[2] - tr, 3.0 * ib[3], 3.0 * ib[4], 3.0 * ib[5]};
+    double* q64 = &t.q64[8 * i];
+    q64[0] = rm;
+    for (int a = 0; a < 6; ++a) q64[1 + a] = q[a];
+    q64[7] = 0.0;
+    t.q32[2 * i + 0] = make_float4(static_cast<float>(rm), static_cast<float>(q[0]), static_cast<float>(q[1]),
+                                   static_cast<float>(q[2]));
+    t.q32[2 * i + 1] = make_float4(static_cast<float>(q[3]), static_cast<float>(q[4]), static_cast<float>(q[5]), 0.0f);
+  }
+}
+
+// Marks each walk target at its first particle: topmost nodes with
+// leaf || count <= tsm (tsm = min(target_size, group_size), which makes the
+// group-root / target nesting of grouped_ttg.h:251-288 a single predicate).
+__global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_nodes, uint32_t tsm, uint32_t* __restrict__ mark) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
+    const bool pred = t.leaf[i] || t.count[i] <= tsm;
+    if (!pred) continue;
+    const bool ppred = (i != 0) && (t.count[t.parent[i]] <= tsm);
+    if (!ppred) mark[t.begin[i]] = i + 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_count(const uint32_t* __restrict__ mark, uint32_t n,
+                                                          uint32_t* __restrict__ tile_counts) {
+  __shared__ uint32_t s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kTreeTile;
+  uint32_t c = 0;
+  for (uint32_t e = threadIdx.x; e < kTreeTile; e += kBlock) {
+    const uint32_t i = base + e;
+    if (i < n && mark[i]) ++c;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (__lane_id() == 0) atomicAdd(&s, c);
+  __syncthreads();
+  if (threadIdx.x == 0) tile_counts[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_write(const uint32_t* __restrict__ mark, uint32_t n,
+                                                          const uint32_t* __restrict__ tile_prefix, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_wc[4];
+  const uint32_t base = blockIdx.x * kTreeTile;
+  const int w = threadIdx.x >> 6, lane = __lane_id();
+  uint32_t vals[kItems];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t i = base + w * (kTreeTile / 4) + j * 64 + lane;
+    vals[j] = i < n ? mark[i] : 0u;
+    cnt += __popcll(__ballot(vals[j] != 0));
+  }
+  if (lane == 0) s_wc[w] = cnt;
+  __syncthreads();
+  uint32_t pos = tile_prefix[blockIdx.x];
+  for (int k = 0; k < w; ++k) pos += s_wc[k];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint64_t m = __ballot(vals[j] != 0);
+    if (vals[j]) out[pos + __popcll(m & lt)] = vals[j] - 1;
+    pos += __popcll(m);
+  }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t cap) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  return static_cast<uint32_t>(g < cap ? g : cap);
+}
+
+}  // namespace
+
+hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, uint32_t max_leaf, uint8_t* L, uint8_t* H,
+                             uint32_t* tile_counts, hipStream_t s) {
+  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
+  hipLaunchKernelGGL(k_tree_count, dim3(tiles), dim3(kBlock), 0, s, keys, n, max_leaf, L, H, tile_counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_t* tile_prefix, uint32_t* totals,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_col_scan, dim3(kLevels + 1), dim3(1024), 0, s, tile_counts, tiles, static_cast<uint32_t>(kTreeCols),
+                     tile_prefix, totals);
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix, const uint32_t* totals,
+                            TreeDev t, hipStream_t s) {
+  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
+  hipLaunchKernelGGL(k_tree_emit, dim3(tiles), dim3(kBlock), 0, s, L, H, n, tile_prefix, totals, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const double* lohi,
+                              const uint32_t* totals, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s) {
+  (void)totals;
+  hipLaunchKernelGGL(k_tree_links, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, keys, n, lohi, n_nodes, n_leaves, t);
+  // bottom-up, one launch per level; each reads its level range on the device
+  for (int d = kDepthCap; d >= 0; --d) {
+    hipLaunchKernelGGL(k_upward, dim3(grid_for(n_nodes, 2048)), dim3(kBlock), 0, s, posm, d, t);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, uint32_t tsm, uint32_t* mark, uint32_t* tile_counts,
+                          uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets, hipStream_t s) {
+  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
+  hipError_t e = launch_fill_u32(mark, 0u, n, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, tsm, mark);
+  hipLaunchKernelGGL(k_compact_count, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_counts);
+  hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, 1u, tile_prefix, n_targets);
+  hipLaunchKernelGGL(k_compact_write, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_prefix, targets);
+  return hipGetLastError();
+}
+
+}  // namespace bhk

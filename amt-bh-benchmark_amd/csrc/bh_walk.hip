@@ -1,0 +1,542 @@
+// The grouped force walk on CDNA4: one wavefront per walk target.
+//
+// Reference semantics:
+//   targets           grouped_ttg.h:253 (topmost nodes with leaf || count <= target_size)
+//   MAC               multipole.h:130-153: accept iff theta^2 * near2(com) > (2*hw)^2,
+//                     near2 against the FP64 bounding box of the target's particles
+//   lists             grouped_ttg.h:223-249: accepted nodes -> quadrupole cells (or
+//                     monopoles without quadrupole), opened leaves -> their particles
+//                     (the target's own included), opened interior nodes -> children
+//   P2P               grouped_ttg.h:124-163: a += m d (r^2 + eps^2)^(-3/2), r = 0 masked
+//                     when eps2 == 0
+//   P2C               multipole.h:171-200: monopole + traceless quadrupole
+//   kick/drift        integrate.h:15-16 via integrate_ttg.h:119-122, fused
+//
+// Each target walks the tree from the root on its own. That gives exactly
+// the interaction list of the reference's shared group walk: its shared
+// accept / open-for-all decisions at a group node imply the same decision
+// for every target box inside that node's box (near2 and far2 are monotone
+// in the box, also in floating point), and a target opens whatever it does
+// not accept (grouped_ttg.h:42-51).
+//
+// Per iteration a wavefront pops up to 64 nodes of the top frame of its
+// LDS stack (frame d holds child ranges at tree depth d), tests them with
+// 64 lanes in FP64 - bit-identical to the reference's decisions - and
+// compacts the outcomes with ballots: children ranges into frame d+1,
+// accepted cells into the P2C ring, opened leaves' particles into the P2P
+// ring. Ring entries are staged once per target as (source - target centre)
+// in FP64, then rounded to the walk precision, so FP32 arithmetic never sees
+// absolute coordinates. Whenever a ring holds 64 entries they are evaluated:
+// lane = (target particle p, slice s); with T particles per target each of
+// the S = 64 / T slices takes every S-th entry, so small leaves still use
+// the whole wavefront. FP32 batches are accumulated per lane with Kahan
+// compensation; slices are combined in FP64.
+//
+// No MFMA: the walk is not a dense contraction.
+#include "bh_internal.h"
+
+namespace bhk {
+
+namespace {
+
+constexpr int kRing = 128;  // ring capacity (entries); evaluated in batches of 64
+
+template <typename Real>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+  using T = float4;
+};
+template <>
+struct Vec4T<double> {
+  using T = double4;
+};
+
+template <typename Real>
+struct WaveLds {
+  using V4 = typename Vec4T<Real>::T;
+  uint32_t fr[kLevels][kWave];  // child ranges: (first << 3) | (count - 1)
+  uint32_t fr_n[kLevels];
+  uint32_t fr_cur[kLevels];
+  V4 pp[kRing];       // x, y, z (relative to the target centre), m
+  V4 pc[kRing * 3];   // {x, y, z, m}, {Qxx, Qyy, Qzz, Qxy}, {Qxz, Qyz, 0, 0}
+};
+
+__device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double rsq(double x) { return rsqrt(x); }
+__device__ inline float fmaR(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double fmaR(double a, double b, double c) { return fma(a, b, c); }
+
+// Per-lane acceleration accumulator: Kahan-compensated in FP32, plain in FP64.
+template <typename Real>
+struct Acc {
+  float x = 0.f, y = 0.f, z = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  __device__ inline void add1(float& s, float& c, float b) {
+    const float yv = b - c;
+    const float t = s + yv;
+    c = (t - s) - yv;
+    s = t;
+  }
+  __device__ inline void add(float bx, float by, float bz) {
+    add1(x, cx, bx);
+    add1(y, cy, by);
+    add1(z, cz, bz);
+  }
+  __device__ inline double sx() const { return static_cast<double>(x) - static_cast<double>(cx); }
+  __device__ inline double sy() const { return static_cast<double>(y) - static_cast<double>(cy); }
+  __device__ inline double sz() const { return static_cast<double>(z) - static_cast<double>(cz); }
+};
+template <>
+struct Acc<double> {
+  double x = 0.0, y = 0.0, z = 0.0;
+  __device__ inline void add(double bx, double by, double bz) {
+    x += bx;
+    y += by;
+    z += bz;
+  }
+  __device__ inline double sx() const { return x; }
+  __device__ inline double sy() const { return y; }
+  __device__ inline double sz() const { return z; }
+};
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = __lane_id();
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+__device__ inline uint64_t lanemask_lt() {
+  const int lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Largest lane j in [0, limit) with excl_j <= e (excl non-decreasing over lanes).
+__device__ inline int find_lane(uint32_t excl, uint32_t e, int limit) {
+  int j = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const int c = j + step;
+    const uint32_t ec = __shfl(excl, c < 64 ? c : 63);
+    if (c < limit && ec <= e) j = c;
+  }
+  return j;
+}
+
+template <typename Real, bool kMask>
+__device__ inline void eval_pp(const WaveLds<Real>& L, uint32_t head, uint32_t cnt, int s, int S, Real tx, Real ty, Real tz,
+                               Real eps2, Acc<Real>& acc) {
+  using V4 = typename Vec4T<Real>::T;
+  Real bx = 0, by = 0, bz = 0;
+  for (uint32_t j = s; j < cnt; j += S) {
+    const V4 e = L.pp[(head + j) & (kRing - 1)];
+    const Real dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+    const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
+    Real inv = rsq(r2);
+    if (kMask) inv = r2 > Real(0) ? inv : Real(0);
+    const Real s3 = e.w * inv * inv * inv;
+    bx = fmaR(s3, dx, bx);
+    by = fmaR(s3, dy, by);
+    bz = fmaR(s3, dz, bz);
+  }
+  acc.add(bx, by, bz);
+}
+
+template <typename Real>
+__device__ inline void eval_pc(const WaveLds<Real>& L, uint32_t head, uint32_t cnt, int s, int S, Real tx, Real ty, Real tz,
+                               Real eps2, Acc<Real>& acc) {
+  using V4 = typename Vec4T<Real>::T;
+  Real bx = 0, by = 0, bz = 0;
+  for (uint32_t j = s; j < cnt; j += S) {
+    const uint32_t k = (head + j) & (kRing - 1);
+    const V4 e0 = L.pc[3 * k + 0];
+    const V4 e1 = L.pc[3 * k + 1];
+    const V4 e2 = L.pc[3 * k + 2];
+    const Real dx = e0.x - tx, dy = e0.y - ty, dz = e0.z - tz;
+    const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
+    const Real inv = rsq(r2);
+    const Real inv2 = inv * inv;
+    const Real inv3 = inv * inv2;
+    // Q.d with Q = {xx, yy, zz, xy, xz, yz} = {e1.x, e1.y, e1.z, e1.w, e2.x, e2.y}
+    const Real qx = fmaR(e1.x, dx, fmaR(e1.w, dy, e2.x * dz));
+    const Real qy = fmaR(e1.w, dx, fmaR(e1.y, dy, e2.y * dz));
+    const Real qz = fmaR(e2.x, dx, fmaR(e2.y, dy, e1.z * dz));
+    const Real dqd = fmaR(dx, qx, fmaR(dy, qy, dz * qz));
+    const Real inv5 = inv3 * inv2;
+    const Real inv7 = inv5 * inv2;
+    const Real f = fmaR(Real(2.5) * dqd, inv7, e0.w * inv3);  // M/r^3 + 2.5 dQd/r^7
+    bx = fmaR(f, dx, fmaR(-inv5, qx, bx));
+    by = fmaR(f, dy, fmaR(-inv5, qy, by));
+    bz = fmaR(f, dz, fmaR(-inv5, qz, bz));
+  }
+  acc.add(bx, by, bz);
+}
+
+template <typename Real>
+__device__ inline typename Vec4T<Real>::T make_entry(double x, double y, double z, double m) {
+  using V4 = typename Vec4T<Real>::T;
+  V4 v;
+  v.x = static_cast<Real>(x);
+  v.y = static_cast<Real>(y);
+  v.z = static_cast<Real>(z);
+  v.w = static_cast<Real>(m);
+  return v;
+}
+
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// One target (node tnode): the full walk for each chunk of <= 64 of its
+// particles, then reduction, output and kick/drift.
+template <typename Real, bool kMask>
+__device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode) {
+  const int lane = __lane_id();
+  const uint32_t tb = uniform(a.begin[tnode]);
+  const uint32_t T = uniform(a.count[tnode]);
+  const double4 tcv = a.ctr[tnode];
+  const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
+  // FP64 bounding box of the target's particles (multipole.h:115-128)
+  const double inf = __builtin_huge_val();
+  double blo0 = inf, blo1 = inf, blo2 = inf, bhi0 = -inf, bhi1 = -inf, bhi2 = -inf;
+  for (uint32_t p = lane; p < T; p += kWave) {
+    const double4 q = a.posm[tb + p];
+    blo0 = fmin(blo0, q.x);
+    blo1 = fmin(blo1, q.y);
+    blo2 = fmin(blo2, q.z);
+    bhi0 = fmax(bhi0, q.x);
+    bhi1 = fmax(bhi1, q.y);
+    bhi2 = fmax(bhi2, q.z);
+  }
+  blo0 = wave_min(blo0);
+  blo1 = wave_min(blo1);
+  blo2 = wave_min(blo2);
+  bhi0 = wave_max(bhi0);
+  bhi1 = wave_max(bhi1);
+  bhi2 = wave_max(bhi2);
+  const double theta2 = a.theta2;
+  const Real eps2 = static_cast<Real>(a.eps2);
+  const uint64_t lt = lanemask_lt();
+
+  for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
+    const uint32_t tc = T - c0 < kWave ? T - c0 : kWave;
+    const int S = kWave / static_cast<int>(tc);
+    const bool active = lane < static_cast<int>(tc) * S;
+    const int p = lane % static_cast<int>(tc), s = lane / static_cast<int>(tc);
+    const uint32_t pidx = tb + c0 + p;
+    Real tx = 0, ty = 0, tz = 0;
+    {
+      const double4 q = a.posm[pidx];
+      tx = static_cast<Real>(q.x - tcx);
+      ty = static_cast<Real>(q.y - tcy);
+      tz = static_cast<Real>(q.z - tcz);
+    }
+    Acc<Real> acc;
+    uint32_t pp_head = 0, pp_cnt = 0, pc_head = 0, pc_cnt = 0;
+    uint32_t n_pp = 0, n_pc = 0;
+
+    if (lane == 0) {
+      L.fr[0][0] = 0u;  // node 0, one node
+      L.fr_n[0] = 1;
+      L.fr_cur[0] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    int depth = 0;
+    while (depth >= 0) {
+      const uint32_t nr = uniform(L.fr_n[depth]);
+      const uint32_t cur = uniform(L.fr_cur[depth]);
+      if (cur >= nr) {
+        --depth;
+        continue;
+      }
+      // pop whole child ranges, up to 64 nodes
+      const uint32_t rr = cur + lane;
+      const bool rv = rr < nr;
+      const uint32_t packed = rv ? L.fr[depth][rr] : 0u;
+      const uint32_t rcnt = rv ? (packed & 7u) + 1u : 0u;
+      const uint32_t rfirst = packed >> 3;
+      const uint32_t incl = wave_incl_scan(rcnt);
+      const uint64_t tm = __ballot(rv && incl <= kWave);
+      const int k = __popcll(tm);
+      const uint32_t total = uniform(__shfl(incl, k - 1));
+      const uint32_t excl = incl - rcnt;
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) L.fr_cur[depth] = cur + k;
+      const bool nv = static_cast<uint32_t>(lane) < total;
+      const int j = find_lane(excl, static_cast<uint32_t>(lane), k);
+      const uint32_t node = __shfl(rfirst, j) + (static_cast<uint32_t>(lane) - __shfl(excl, j));
+      // MAC in FP64, reference operation order
+      bool accept = false, isleaf = false;
+      uint32_t lfirst = 0, lcnt = 0;
+      double4 mac = make_double4(0, 0, 0, 0);
+      if (nv) {
+        mac = a.mac[node];
+        const uint2 lk = a.link[node];
+        const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
+        const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
+        const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
+        const double near2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+        accept = theta2 * near2 > mac.w;
+        isleaf = (lk.y >> 31) != 0;
+        lfirst = lk.x;
+        lcnt = lk.y & 0x7fffffffu;
+      }
+      const bool cellf = nv && accept;
+      const bool leaff = nv && !accept && isleaf;
+      // a node at depth 21 is always a leaf; the guard keeps the frame stack in bounds regardless
+      const bool openf = nv && !accept && !isleaf && depth + 1 < kLevels;
+      // children of opened interior nodes -> frame depth + 1
+      const uint64_t om = __ballot(openf);
+      if (om) {
+        const uint32_t pos = __popcll(om & lt);
+        if (openf) L.fr[depth + 1][pos] = (lfirst << 3) | (lcnt - 1u);
+        if (lane == 0) {
+          L.fr_n[depth + 1] = __popcll(om);
+          L.fr_cur[depth + 1] = 0;
+        }
+      }
+      // accepted nodes
+      const uint64_t cm = __ballot(cellf);
+      if (cm) {
+        const uint32_t nc = __popcll(cm);
+        const uint32_t pos = __popcll(cm & lt);
+        if (a.use_quad) {
+          if (cellf) {
+            const uint32_t slot = (pc_head + pc_cnt + pos) & (kRing - 1);
+            if constexpr (sizeof(Real) == 4) {
+              const float4 q0 = a.q32[2 * node + 0], q1 = a.q32[2 * node + 1];
+              L.pc[3 * slot + 0] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, q0.x);
+              L.pc[3 * slot + 1] = make_float4(q0.y, q0.z, q0.w, q1.x);
+              L.pc[3 * slot + 2] = make_float4(q1.y, q1.z, 0.f, 0.f);
+            } else {
+              const double* q = &a.q64[8 * node];
+              L.pc[3 * slot + 0] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, q[0]);
+              L.pc[3 * slot + 1] = make_double4(q[1], q[2], q[3], q[4]);
+              L.pc[3 * slot + 2] = make_double4(q[5], q[6], 0.0, 0.0);
+            }
+          }
+          pc_cnt += nc;
+          n_pc += nc;
+          __builtin_amdgcn_wave_barrier();
+          if (pc_cnt >= kWave) {
+            if (active) eval_pc<Real>(L, pc_head, kWave, s, S, tx, ty, tz, eps2, acc);
+            pc_head += kWave;
+            pc_cnt -= kWave;
+          }
+        } else {
+          if (cellf) {
+            const uint32_t slot = (pp_head + pp_cnt + pos) & (kRing - 1);
+            const double m = sizeof(Real) == 4 ? static_cast<double>(a.q32[2 * node].x) : a.q64[8 * node];
+            L.pp[slot] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, m);
+          }
+          pp_cnt += nc;
+          n_pp += nc;
+          __builtin_amdgcn_wave_barrier();
+          if (pp_cnt >= kWave) {
+            if (active) eval_pp<Real, kMask>(L, pp_head, kWave, s, S, tx, ty, tz, eps2, acc);
+            pp_head += kWave;
+            pp_cnt -= kWave;
+          }
+        }
+      }
+      // particles of opened leaves
+      const uint64_t lm = __ballot(leaff);
+      if (lm) {
+        const uint32_t lc = leaff ? lcnt : 0u;
+        const uint32_t lincl = wave_incl_scan(lc);
+        const uint32_t lexcl = lincl - lc;
+        const uint32_t ltot = uniform(__shfl(lincl, 63));
+        for (uint32_t b = 0; b < ltot; b += kWave) {
+          const uint32_t e = b + lane;
+          const int jl = find_lane(lexcl, e, kWave);
+          const uint32_t q = __shfl(lfirst, jl) + (e - __shfl(lexcl, jl));
+          const uint32_t chunk = ltot - b < kWave ? ltot - b : kWave;
+          if (e < ltot) {
+            const double4 sp = a.posm[q];
+            L.pp[(pp_head + pp_cnt + lane) & (kRing - 1)] = make_entry<Real>(sp.x - tcx, sp.y - tcy, sp.z - tcz, sp.w);
+          }
+          pp_cnt += chunk;
+          n_pp += chunk;
+          __builtin_amdgcn_wave_barrier();
+          if (pp_cnt >= kWave) {
+            if (active) eval_pp<Real, kMask>(L, pp_head, kWave, s, S, tx, ty, tz, eps2, acc);
+            pp_head += kWave;
+            pp_cnt -= kWave;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (om) ++depth;
+    }
+    if (active) {
+      if (pp_cnt) eval_pp<Real, kMask>(L, pp_head, pp_cnt, s, S, tx, ty, tz, eps2, acc);
+      if (pc_cnt) eval_pc<Real>(L, pc_head, pc_cnt, s, S, tx, ty, tz, eps2, acc);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // combine slices in FP64 (the P2C ring is free now)
+    double* red = reinterpret_cast<double*>(L.pc);
+    if (active) {
+      red[3 * lane + 0] = acc.sx();
+      red[3 * lane + 1] = acc.sy();
+      red[3 * lane + 2] = acc.sz();
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < static_cast<int>(tc)) {
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      for (int k2 = 0; k2 < S; ++k2) {
+        const int src = k2 * static_cast<int>(tc) + lane;
+        ax += red[3 * src + 0];
+        ay += red[3 * src + 1];
+        az += red[3 * src + 2];
+      }
+      const double gx = a.G * ax, gy = a.G * ay, gz = a.G * az;
+      const uint32_t k3 = tb + c0 + lane;
+      if (a.acc) {
+        a.acc[3 * uint64_t(k3) + 0] = gx;
+        a.acc[3 * uint64_t(k3) + 1] = gy;
+        a.acc[3 * uint64_t(k3) + 2] = gz;
+      }
+      if (a.n_pp) a.n_pp[k3] = n_pp;
+      if (a.n_pc) a.n_pc[k3] = n_pc;
+      if (a.parts) {
+        // kick_one / drift_one (integrate.h:15-16): v += a*dt; x += v*dt
+        bh_particle& pt = a.parts[k3];
+        const double vx = pt.vel[0] + gx * a.kick_dt;
+        const double vy = pt.vel[1] + gy * a.kick_dt;
+        const double vz = pt.vel[2] + gz * a.kick_dt;
+        pt.vel[0] = vx;
+        pt.vel[1] = vy;
+        pt.vel[2] = vz;
+        if (a.drift_dt != 0.0) {
+          pt.pos[0] = pt.pos[0] + vx * a.drift_dt;
+          pt.pos[1] = pt.pos[1] + vy * a.drift_dt;
+          pt.pos[2] = pt.pos[2] + vz * a.drift_dt;
+        }
+      }
+    }
+    if (lane == 0) {
+      atomicAdd(&a.totals[0], static_cast<unsigned long long>(n_pp) * tc);
+      atomicAdd(&a.totals[1], static_cast<unsigned long long>(n_pc) * tc);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Block b runs on XCD b % 8; give each XCD a contiguous run of targets so
+// neighbouring targets share that XCD's L2.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
+  const uint32_t per = grid / 8;
+  return (b % 8) * per + b / 8;
+}
+
+template <typename Real, int kWaves, bool kMask>
+__global__ __launch_bounds__(kWaves * 64) void k_walk(WalkArgs a) {
+  __shared__ WaveLds<Real> lds[kWaves];
+  const int w = threadIdx.x >> 6;
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t t = blk * kWaves + w;
+  if (t >= a.n_targets) return;
+  walk_target<Real, kMask>(a, lds[w], a.targets[t]);
+}
+
+// brute_force_accel (brute_force.h:15-28) for targets tidx[0..m) over a
+// slice of the sources; partial sums per slice, reduced in slice order.
+__global__ __launch_bounds__(256) void k_direct(const double4* __restrict__ src, uint32_t n, const uint32_t* __restrict__ tidx,
+                                                uint32_t m, double eps2, double* __restrict__ partial, uint32_t splits) {
+  __shared__ double4 s_src[256];
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const uint32_t lo = static_cast<uint32_t>(uint64_t(n) * split / splits);
+  const uint32_t hi = static_cast<uint32_t>(uint64_t(n) * (split + 1) / splits);
+  const bool ok = k < m;
+  const uint32_t i = ok ? (tidx ? tidx[k] : k) : 0xffffffffu;
+  double4 t = make_double4(0, 0, 0, 0);
+  if (ok) t = src[i];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (uint32_t b = lo; b < hi; b += 256) {
+    __syncthreads();
+    if (b + threadIdx.x < hi) s_src[threadIdx.x] = src[b + threadIdx.x];
+    __syncthreads();
+    const uint32_t e = hi - b < 256 ? hi - b : 256;
+    for (uint32_t jj = 0; jj < e; ++jj) {
+      if (b + jj == i) continue;
+      const double4 q = s_src[jj];
+      const double dx = q.x - t.x, dy = q.y - t.y, dz = q.z - t.z;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double ds2 = d2 + eps2;
+      const double ds = sqrt(ds2);
+      const double sc = q.w * (1.0 / (ds2 * ds));
+      ax += dx * sc;
+      ay += dy * sc;
+      az += dz * sc;
+    }
+  }
+  if (ok) {
+    double* o = &partial[(uint64_t(split) * m + k) * 3];
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = az;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_direct_reduce(const double* __restrict__ partial, uint32_t m, uint32_t splits,
+                                                       double G, double* __restrict__ acc) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= m) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (uint32_t s = 0; s < splits; ++s) {
+    const double* o = &partial[(uint64_t(s) * m + k) * 3];
+    a0 += o[0];
+    a1 += o[1];
+    a2 += o[2];
+  }
+  acc[3 * uint64_t(k) + 0] = G * a0;
+  acc[3 * uint64_t(k) + 1] = G * a1;
+  acc[3 * uint64_t(k) + 2] = G * a2;
+}
+
+template <typename Real, int kWaves>
+hipError_t launch_walk_t(const WalkArgs& a, hipStream_t s) {
+  uint32_t blocks = (a.n_targets + kWaves - 1) / kWaves;
+  blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
+  if (a.eps2 > 0.0)
+    hipLaunchKernelGGL((k_walk<Real, kWaves, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_walk<Real, kWaves, true>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_walk(const WalkArgs& a, int fp64, hipStream_t s) {
+  if (a.n_targets == 0) return hipSuccess;
+  return fp64 ? launch_walk_t<double, 2>(a, s) : launch_walk_t<float, 2>(a, s);
+}
+
+uint32_t direct_splits(uint32_t n, uint32_t m) {
+  const uint32_t tblocks = (m + 255) / 256;
+  uint32_t splits = 2048 / (tblocks ? tblocks : 1);
+  if (splits < 1) splits = 1;
+  if (splits > 512) splits = 512;
+  while (splits > 1 && n / splits < 256) splits >>= 1;
+  return splits;
+}
+
+hipError_t launch_direct(const double4* src, uint32_t n, const uint32_t* tidx, uint32_t m, double eps2, double G,
+                         double* partial, uint32_t splits, double* acc, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_direct, dim3((m + 255) / 256, splits), dim3(256), 0, s, src, n, tidx, m, eps2, partial, splits);
+  hipLaunchKernelGGL(k_direct_reduce, dim3((m + 255) / 256), dim3(256), 0, s, partial, m, splits, G, acc);
+  return hipGetLastError();
+}
+
+}  // namespace bhk

@@ -1,0 +1,155 @@
+/*
+ * MI355X Barnes-Hut force path: C ABI.
+ *
+ * Drop-in for the reference's grouped, single-rank force path
+ * (devreal/AMT-BH-benchmark, ttg-parsec/src): per-step Morton sort, flat
+ * octree build with multipoles, grouped force walk, leapfrog kick/drift.
+ * Plain C types only; all device state is owned by a bh_ctx.
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   step_particles          ttg-parsec/src/integrate_ttg.h:111-193 (grouped branch, 1 rank)
+ *   build_flat_tree         ttg-parsec/src/flat_build_ttg.h:276-281
+ *   run_grouped_pass        ttg-parsec/src/grouped_ttg.h:297-323
+ *   brute_force_accel       ttg-parsec/src/brute_force.h:15-28
+ *   kinetic/potential_energy ttg-parsec/src/integrate.h:27-43
+ */
+#ifndef BH_GPU_H
+#define BH_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* bh::Particle (particle.h:7-24): id@0, pad, mass@8, pos@16, vel@40; 64 bytes. */
+typedef struct bh_particle {
+  uint32_t id;
+  uint32_t pad;
+  double mass;
+  double pos[3];
+  double vel[3];
+} bh_particle;
+
+/* bh::ForceParams (traversal_ttg.h:136-141). eps2 is the squared softening. */
+typedef struct bh_force_params {
+  double theta;
+  double eps2;
+  double G;
+  int32_t use_quadrupole;
+  int32_t pad;
+} bh_force_params;
+
+/* bh::StepTiming (integrate_ttg.h:69-73), milliseconds; exchange_ms is 0 on one GPU. */
+typedef struct bh_step_timing {
+  double tree_build_ms;
+  double traversal_ms;
+  double exchange_ms;
+} bh_step_timing;
+
+/* bh::WalkNode (flat_tree.h:61-70), 56 bytes; used by bh_tree_export. */
+typedef struct bh_walk_node {
+  double com[3];
+  double mass;
+  double half_width;
+  uint32_t first_child;
+  uint32_t particle_begin;
+  uint32_t particle_count;
+  uint8_t num_children;
+  uint8_t leaf;
+  uint8_t depth;
+  uint8_t pad;
+} bh_walk_node;
+
+/* Status codes. The reference throws (integrate_ttg.h:149-151, main.cc:147);
+ * no exception crosses this ABI: every call returns one of these and
+ * bh_last_error() holds the message. */
+enum bh_status {
+  BH_OK = 0,
+  BH_ERR_INVALID = 1,     /* bad argument (null pointer, group_size 0, ...) */
+  BH_ERR_NONFINITE = 2,   /* a position, velocity or mass is NaN or infinite */
+  BH_ERR_INCOMPLETE = 3,  /* the walk opened a node without particles or children */
+  BH_ERR_OOM = 4,         /* device allocation failed */
+  BH_ERR_HIP = 5,         /* HIP runtime error */
+  BH_ERR_TOO_LARGE = 6,   /* n or max_leaf beyond the supported range */
+  BH_ERR_NO_TREE = 7      /* bh_accel / bh_tree_export before bh_build_tree */
+};
+
+/* Walk arithmetic. FP32: interactions in FP32 with compensated per-particle
+ * accumulation (default). FP64: interactions in FP64. The MAC, tree build,
+ * multipoles and kick/drift are FP64 in both. */
+enum bh_precision { BH_FP32 = 0, BH_FP64 = 1 };
+
+/* Node numbering of bh_tree_export: the reference's (backbone BFS, then each
+ * subtree of <= 4096 particles BFS, flat_build_ttg.h:194-273) or this
+ * library's own level-order BFS. */
+enum bh_layout { BH_LAYOUT_REFERENCE = 0, BH_LAYOUT_BFS = 1 };
+
+typedef struct bh_ctx bh_ctx;
+
+/* Context on HIP device `device`: one stream, grow-only device buffers (the
+ * GroupedWorkspace of integrate_ttg.h:106-109). Not re-entrant: use one
+ * context per host thread. */
+int bh_ctx_create(bh_ctx** out, int device);
+void bh_ctx_destroy(bh_ctx* ctx);
+const char* bh_last_error(const bh_ctx* ctx);
+int bh_set_precision(bh_ctx* ctx, int precision);
+/* Limits: n < 2^30 particles, max_leaf <= 1024. */
+uint64_t bh_max_particles(void);
+
+/* step_particles (integrate_ttg.h:111-114), grouped branch on one rank:
+ * builds the tree of particles[0..n), walks it and applies
+ * v += kick_dt * a; if (drift_dt != 0) x += drift_dt * v
+ * to every particle, and returns them IN MORTON ORDER in `particles` (as the
+ * reference does, integrate_ttg.h:152). Host memory in and out.
+ * group_size must be > 0 (0 selects the reference's per-node path, which
+ * this library does not provide). `timing` may be NULL. */
+int bh_step(bh_ctx* ctx, bh_particle* particles, uint64_t n, const bh_force_params* fp, uint32_t max_leaf,
+            uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt, bh_step_timing* timing);
+
+/* The same on device memory: reads d_in[0..n), writes the Morton-ordered,
+ * updated particles to d_out[0..n). d_in and d_out must not overlap. The work
+ * is ordered on the context's stream; the call returns when it is complete. */
+int bh_step_device(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
+                   uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
+                   bh_step_timing* timing);
+
+/* Particles resident in device memory across steps (the caller's
+ * GroupedWorkspace reuse, main.cc:203-242, without host copies). */
+int bh_state_upload(bh_ctx* ctx, const bh_particle* host, uint64_t n);
+int bh_state_step(bh_ctx* ctx, const bh_force_params* fp, uint32_t max_leaf, uint32_t group_size, uint32_t target_size,
+                  double kick_dt, double drift_dt, bh_step_timing* timing);
+int bh_state_download(bh_ctx* ctx, bh_particle* host, uint64_t n);
+/* Device address of the resident particles (Morton order after a step). */
+bh_particle* bh_state_device_ptr(bh_ctx* ctx);
+
+/* build_flat_tree (flat_build_ttg.h:276-281) of host particles; the tree stays
+ * in the context for bh_tree_export / bh_accel. */
+int bh_build_tree(bh_ctx* ctx, const bh_particle* host, uint64_t n, uint32_t max_leaf, uint32_t group_size);
+/* Sizes of the last built tree; dom4 (nullable) receives centre xyz and half width. */
+int bh_tree_sizes(bh_ctx* ctx, uint64_t* n_nodes, uint64_t* n_groups, double* dom4);
+/* Copies the last built tree out, nodes numbered per `layout` (bh_layout).
+ * keys[n], perm[n] (sorted position -> input index), sorted[n] (particles in
+ * Morton order), nodes[n_nodes], center[3*n_nodes], iab[6*n_nodes]
+ * (xx,yy,zz,xy,xz,yz about the centre of mass), groups[n_groups] (group
+ * roots in the reference's order). Any pointer may be NULL. */
+int bh_tree_export(bh_ctx* ctx, int layout, uint64_t* keys, uint32_t* perm, bh_particle* sorted, bh_walk_node* nodes,
+                   double* center, double* iab, uint32_t* groups);
+/* run_grouped_pass (grouped_ttg.h:297-298) over the last built tree with the
+ * update replaced by output: acc[3*n] = G * acceleration of each particle in
+ * Morton order; n_pp[n] / n_pc[n] = length of its particle(+monopole) list and
+ * its quadrupole-cell count; totals[2] = sum over particles of each. Any
+ * output may be NULL. */
+int bh_accel(bh_ctx* ctx, const bh_force_params* fp, uint32_t target_size, double* acc, uint32_t* n_pp, uint32_t* n_pc,
+             uint64_t* totals);
+
+/* brute_force_accel (brute_force.h:15-28) for the targets idx[0..m) among
+ * host particles[0..n), FP64 on the device; acc[3*m]. idx NULL = all. */
+int bh_direct_sum(bh_ctx* ctx, const bh_particle* host, uint64_t n, const uint32_t* idx, uint64_t m, double eps2, double G,
+                  double* acc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BH_GPU_H */
