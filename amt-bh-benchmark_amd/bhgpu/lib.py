@@ -50,6 +50,7 @@ SIGNATURES = {
     "bh_tree_export": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "bh_accel": ([_P, _P, _U32, _P, _P, _P, _P], _I),
     "bh_direct_sum": ([_P, _P, _U64, _P, _U64, _D, _D, _P], _I),
+    "bh_last_step_stats": ([_P, _P], _I),
 }
 
 _lib = None
@@ -78,6 +79,15 @@ class BHError(RuntimeError):
 
 class StepTiming(ctypes.Structure):
     _fields_ = [("tree_build_ms", ctypes.c_double), ("traversal_ms", ctypes.c_double), ("exchange_ms", ctypes.c_double)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("sort_ms", ctypes.c_double), ("tree_ms", ctypes.c_double), ("targets_ms", ctypes.c_double),
+                ("walk_ms", ctypes.c_double), ("n_pp", ctypes.c_uint64), ("n_pc", ctypes.c_uint64),
+                ("n_nodes", ctypes.c_uint64), ("n_leaves", ctypes.c_uint64), ("n_targets", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def _ptr(a):
@@ -185,6 +195,12 @@ class Context:
         out["domain"] = dom
         return out
 
+    def export_perm(self):
+        """Sorted position -> input index of the last built tree."""
+        perm = np.zeros(self._n, np.uint32)
+        self._check(self._lib.bh_tree_export(self._h, BH_LAYOUT_BFS, None, _ptr(perm), None, None, None, None, None))
+        return perm
+
     def accel(self, fp, target_size):
         """run_grouped_pass on the last tree: (acc, n_pp, n_pc, totals), Morton order."""
         n = self._n
@@ -194,6 +210,11 @@ class Context:
         tot = np.zeros(2, np.uint64)
         self._check(self._lib.bh_accel(self._h, _ptr(_fp(fp)), target_size, _ptr(acc), _ptr(npp), _ptr(npc), _ptr(tot)))
         return acc, npp, npc, tot
+
+    def last_step_stats(self):
+        st = StepStats()
+        self._check(self._lib.bh_last_step_stats(self._h, ctypes.byref(st)))
+        return st
 
     def direct_sum(self, p, idx, eps2, G=1.0):
         p = np.ascontiguousarray(p, dtype=PARTICLE)

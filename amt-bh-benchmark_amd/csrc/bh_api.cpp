@@ -33,7 +33,9 @@ struct DevBuf {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // events: 0 step start, 1 sorted, 2 tree built, 3 targets chosen, 4 walk start, 5 walk end
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool have_stats = false;
   std::string err;
   int precision = BH_FP32;
 
@@ -201,6 +203,7 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n, uint3
   HIP_OK(c, launch_radix_sort(c->keys_a.p, c->vals_a.p, c->keys_b.p, c->vals_b.p, n, c->hist.p, c->dbase.p, c->lookback.p,
                               c->lookback.p + (lb_words - 8), s));
   HIP_OK(c, launch_gather(d_in, d_out, c->vals_a.p, c->posm.p, n, s));
+  HIP_OK(c, hipEventRecord(c->ev[1], s));
   HIP_OK(c, launch_tree_count(c->keys_a.p, n, max_leaf, c->L.p, c->H.p, c->tcounts.p, s));
   HIP_OK(c, launch_tree_scan(c->tcounts.p, tiles, c->tprefix.p, c->misc.p + 8, s));
   HIP_OK(c, hipMemcpyAsync(c->host_misc, c->misc.p, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -239,6 +242,7 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n, uint3
   const TreeDev t = tree_view(c);
   HIP_OK(c, launch_tree_emit(c->L.p, c->H.p, n, c->tprefix.p, c->misc.p + 8, t, s));
   HIP_OK(c, launch_tree_finish(c->keys_a.p, c->posm.p, n, c->lohi.p, c->misc.p + 8, m, leaves, t, s));
+  HIP_OK(c, hipEventRecord(c->ev[2], s));
   c->n = n;
   c->n_nodes = m;
   c->n_leaves = leaves;
@@ -260,8 +264,9 @@ int walk(Ctx* c, const bh_force_params* fp, uint32_t group_size, uint32_t target
   const uint32_t tsm = std::min(target_size, group_size);
   HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, tsm, c->mark.p, c->tcounts.p, c->tprefix.p, c->targets.p,
                            c->misc.p + 40, s));
-  HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
+  HIP_OK(c, hipEventRecord(c->ev[3], s));
+  HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(c, hipStreamSynchronize(s));
   (void)tiles;
   WalkArgs a;
@@ -286,7 +291,10 @@ int walk(Ctx* c, const bh_force_params* fp, uint32_t group_size, uint32_t target
   a.kick_dt = kick_dt;
   a.drift_dt = drift_dt;
   a.use_quad = fp->use_quadrupole != 0;
+  HIP_OK(c, hipEventRecord(c->ev[4], s));
   HIP_OK(c, launch_walk(a, c->precision == BH_FP64, s));
+  HIP_OK(c, hipEventRecord(c->ev[5], s));
+  c->have_stats = true;
   return BH_OK;
 }
 
@@ -303,16 +311,15 @@ int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n,
   if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
   if (n == 0) return BH_OK;
   HIP_OK(c, hipSetDevice(c->device));
+  c->have_stats = false;
   HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
   TRY(build(c, d_in, d_out, static_cast<uint32_t>(n), max_leaf));
   c->group_size = group_size;
-  HIP_OK(c, hipEventRecord(c->ev[1], c->stream));
   TRY(walk(c, fp, group_size, target_size, d_out, kick_dt, drift_dt, nullptr, nullptr, nullptr));
-  HIP_OK(c, hipEventRecord(c->ev[2], c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   if (timing) {
-    timing->tree_build_ms = elapsed(c->ev[0], c->ev[1]);
-    timing->traversal_ms = elapsed(c->ev[1], c->ev[2]);
+    timing->tree_build_ms = elapsed(c->ev[0], c->ev[2]);
+    timing->traversal_ms = elapsed(c->ev[2], c->ev[5]);
     timing->exchange_ms = 0.0;
   }
   return BH_OK;
@@ -567,6 +574,8 @@ int bh_build_tree(bh_ctx* h, const bh_particle* host, uint64_t n, uint32_t max_l
   TRY(ensure(c, c->tmp_in, n, "particles"));
   TRY(ensure(c, c->pb, n, "particles"));
   HIP_OK(c, hipMemcpyAsync(c->tmp_in.p, host, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
+  c->have_stats = false;
+  HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
   TRY(build(c, c->tmp_in.p, c->pb.p, static_cast<uint32_t>(n), max_leaf));
   c->group_size = group_size;
   c->state_n = 0;
@@ -707,6 +716,29 @@ int bh_accel(bh_ctx* h, const bh_force_params* fp, uint32_t target_size, double*
     totals[0] = t2[0];
     totals[1] = t2[1];
   }
+  return BH_OK;
+}
+
+int bh_last_step_stats(bh_ctx* h, bh_step_stats* st) {
+  Ctx* c = as(h);
+  if (!c || !st) return BH_ERR_INVALID;
+  if (!c->have_stats) {
+    set_err(c, "no step or walk has run on this context");
+    return BH_ERR_NO_TREE;
+  }
+  HIP_OK(c, hipSetDevice(c->device));
+  unsigned long long t2[2] = {0, 0};
+  HIP_OK(c, hipMemcpyAsync(t2, c->totals.p, sizeof t2, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  st->sort_ms = elapsed(c->ev[0], c->ev[1]);
+  st->tree_ms = elapsed(c->ev[1], c->ev[2]);
+  st->targets_ms = elapsed(c->ev[2], c->ev[3]);
+  st->walk_ms = elapsed(c->ev[4], c->ev[5]);
+  st->n_pp = t2[0];
+  st->n_pc = t2[1];
+  st->n_nodes = c->n_nodes;
+  st->n_leaves = c->n_leaves;
+  st->n_targets = c->host_misc[40];
   return BH_OK;
 }
 

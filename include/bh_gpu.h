@@ -143,6 +143,23 @@ int bh_tree_export(bh_ctx* ctx, int layout, uint64_t* keys, uint32_t* perm, bh_p
 int bh_accel(bh_ctx* ctx, const bh_force_params* fp, uint32_t target_size, double* acc, uint32_t* n_pp, uint32_t* n_pc,
              uint64_t* totals);
 
+/* Per-phase device times of the last step or bh_accel call, from HIP events
+ * recorded on the context's stream (the reference's StepTiming split,
+ * integrate_ttg.h:69-73, refined), and the work the walk did. Has no
+ * reference counterpart beyond StepTiming; it exists for measurement. */
+typedef struct bh_step_stats {
+  double sort_ms;     /* bounds, Morton keys, radix sort, gather to Morton order */
+  double tree_ms;     /* topology, geometry, multipoles */
+  double targets_ms;  /* walk-target selection */
+  double walk_ms;     /* the walk kernel alone, fused kick/drift included */
+  uint64_t n_pp;      /* particle (and monopole) interactions evaluated */
+  uint64_t n_pc;      /* quadrupole-cell interactions evaluated */
+  uint64_t n_nodes;
+  uint64_t n_leaves;
+  uint64_t n_targets;
+} bh_step_stats;
+int bh_last_step_stats(bh_ctx* ctx, bh_step_stats* stats);
+
 /* brute_force_accel (brute_force.h:15-28) for the targets idx[0..m) among
  * host particles[0..n), FP64 on the device; acc[3*m]. idx NULL = all. */
 int bh_direct_sum(bh_ctx* ctx, const bh_particle* host, uint64_t n, const uint32_t* idx, uint64_t m, double eps2, double G,

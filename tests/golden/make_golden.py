@@ -1,0 +1,67 @@
+"""Writes tests/golden/*.npz from the reference's own functions.
+
+Runs in the build container only (needs oracle/_ref/libbhref.so, compiled by
+oracle/Makefile from the reference sources). Each fixture holds the input
+particles and what the reference-compiled code returns for them: sorted keys,
+permutation, the FlatTree node arrays (reference layout), group roots, and
+per-particle accelerations and list lengths of a per-target walk built from
+the reference's MAC and force functions.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "amt-bh-benchmark_amd"))
+sys.path.insert(0, REPO)
+
+from bhgpu import ic, layout  # noqa: E402
+
+import oracle  # noqa: E402
+
+
+def read_csv(path):
+    rows = np.loadtxt(path, delimiter=",", ndmin=2)
+    p = np.zeros(len(rows), layout.PARTICLE)
+    p["id"] = rows[:, 0].astype(np.uint32)
+    p["mass"] = rows[:, 1]
+    p["pos"] = rows[:, 2:5]
+    p["vel"] = rows[:, 5:8]
+    return p
+
+
+# name -> (particles, max_leaf, group_size, target_size, theta, eps2)
+CASES = {
+    "two_body": (lambda: read_csv(os.path.join(HERE, "two_body.csv")), 8, 256, 32, 0.5, 1e-4),
+    "shared_ic_20": (lambda: read_csv(os.path.join(HERE, "shared_ic_20.csv")), 8, 256, 32, 0.5, 1e-4),
+    "box2000": (lambda: ic.random_box(2000, 4, 10.0), 8, 64, 32, 0.5, 1e-6),
+    "box6000": (lambda: ic.random_box(6000, 5, 10.0), 16, 64, 32, 0.5, 1e-6),
+    "plummer5000": (lambda: ic.plummer(5000, 7), 4, 32, 16, 0.6, 1e-4),
+}
+
+
+def main():
+    R = oracle.Reference()
+    for name, (gen, ml, gs, ts, theta, eps2) in CASES.items():
+        p = gen()
+        t = R.build(p, ml, gs)
+        e = t.export()
+        acc, npp, npc = t.target_walk(theta, eps2, 1.0, True, ts)
+        np.savez_compressed(
+            os.path.join(HERE, f"{name}.npz"),
+            particles=p.view(np.uint8).reshape(len(p), 64),
+            params=np.array([ml, gs, ts], np.int64),
+            force=np.array([theta, eps2, 1.0]),
+            domain=R.domain(p),
+            keys=e["keys"], perm=e["perm"], nodes=e["nodes"].view(np.uint8).reshape(len(e["nodes"]), 56),
+            center=e["center"], iab=e["iab"], groups=e["groups"], acc=acc, n_pp=npp, n_pc=npc,
+        )
+        print(f"{name}: n={len(p)} nodes={len(e['nodes'])} groups={len(e['groups'])}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+"""GPU parity through the C ABI against the oracle and the reference fixtures.
+
+Bars: keys, permutation, sorted particles, node arrays, centres, second
+moments and group roots bit-exact (reference node layout); per-particle
+interaction-list lengths exact (so the MAC decisions are identical);
+accelerations within 1e-12 (FP64 walk) and 1e-5 (FP32 walk with compensated
+accumulation), global-normalised, against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import golden_io
+from bhgpu import ic, layout
+from bhgpu import lib as bl
+
+pytestmark = pytest.mark.gpu
+
+TREE_FIELDS = ("keys", "perm", "particles", "nodes", "center", "iab", "groups")
+TOL = {bl.BH_FP64: 1e-12, bl.BH_FP32: 1e-5}
+
+
+def rel_err(a, b):
+    den = (b ** 2).sum()
+    return float(np.sqrt(((a - b) ** 2).sum() / den)) if den > 0 else float(np.abs(a - b).max())
+
+
+def check_tree(gpu, orc, p, ml, gs):
+    print(f"tree n={len(p)} max_leaf={ml} group_size={gs}", flush=True)
+    gpu.build_tree(p, ml, gs)
+    ge = gpu.export()
+    oe = orc.build(p, ml, gs).export()
+    for k in TREE_FIELDS:
+        assert golden_io.bytes_equal(ge[k], oe[k]), k
+    assert np.array_equal(ge["domain"], oe["domain"])
+    return ge
+
+
+CASES = [(1, 8, 64, 1), (2, 8, 64, 1), (9, 8, 64, 2), (200, 4, 16, 1), (2000, 8, 64, 4), (4096, 16, 256, 3),
+         (4097, 16, 256, 3), (20000, 16, 64, 5), (20000, 16, 8192, 5), (5000, 1, 64, 6), (3000, 0, 64, 7),
+         (70000, 32, 256, 9)]
+
+
+@pytest.mark.parametrize("n,ml,gs,seed", CASES)
+def test_tree_bit_exact(gpu, orc, n, ml, gs, seed):
+    check_tree(gpu, orc, ic.random_box(n, seed, 10.0), ml, gs)
+
+
+def test_tree_plummer_and_uniform(gpu, orc):
+    check_tree(gpu, orc, ic.plummer(65536, 1), 16, 256)
+    check_tree(gpu, orc, ic.uniform_cube(50000, 2), 8, 256)
+
+
+def test_tree_duplicates_and_degenerate(gpu, orc):
+    p = ic.random_box(3000, 3, 10.0)
+    p["pos"][:1000] = p["pos"][17]
+    check_tree(gpu, orc, p, 4, 16)
+    q = ic.random_box(50, 3, 10.0)
+    q["pos"][:] = q["pos"][0]
+    check_tree(gpu, orc, q, 4, 16)
+    r = ic.random_box(500, 4, 10.0)
+    r["pos"][:, 2] = 1.5  # flat in z
+    check_tree(gpu, orc, r, 8, 64)
+
+
+@pytest.mark.parametrize("name", golden_io.NAMES)
+def test_reference_fixtures(gpu, name):
+    g = golden_io.load(name)
+    print(f"fixture {name}", flush=True)
+    gpu.build_tree(g["particles"], g["max_leaf"], g["group_size"])
+    ge = gpu.export()
+    for k in ("keys", "perm", "nodes", "center", "iab", "groups"):
+        assert golden_io.bytes_equal(ge[k], g[k]), k
+    for prec in (bl.BH_FP64, bl.BH_FP32):
+        gpu.set_precision(prec)
+        acc, npp, npc, _ = gpu.accel(g["fp"], g["target_size"])
+        assert np.array_equal(npp, g["n_pp"]) and np.array_equal(npc, g["n_pc"])
+        assert rel_err(acc, g["acc"]) < TOL[prec]
+    gpu.set_precision(bl.BH_FP32)
+
+
+WALKS = [(2000, 8, 64, 32, 0.5, 1e-6, True), (2000, 8, 64, 32, 0.5, 1e-6, False), (2000, 8, 64, 32, 0.5, 0.0, True),
+         (20000, 16, 64, 32, 0.5, 1e-6, True), (20000, 16, 8192, 32, 0.6, 1e-6, True), (5000, 1, 64, 32, 0.5, 1e-6, True),
+         (5000, 4, 256, 100, 0.5, 1e-6, True), (3000, 8, 64, 0, 0.4, 1e-6, True), (3000, 2, 512, 300, 0.7, 1e-4, True),
+         (200, 4, 16, 32, 0.0, 1e-6, True)]
+
+
+@pytest.mark.parametrize("n,ml,gs,ts,theta,eps2,quad", WALKS)
+def test_walk_parity(gpu, orc, n, ml, gs, ts, theta, eps2, quad):
+    p = ic.random_box(n, 11, 10.0)
+    fp = layout.force_params(theta, eps2, 1.0, quad)
+    ot = orc.build(p, ml, gs)
+    oa, opp, opc, otot = ot.grouped_pass(fp, ts)
+    gpu.build_tree(p, ml, gs)
+    for prec in (bl.BH_FP64, bl.BH_FP32):
+        print(f"walk n={n} ml={ml} gs={gs} ts={ts} theta={theta} eps2={eps2} quad={quad} prec={prec}", flush=True)
+        gpu.set_precision(prec)
+        ga, gpp, gpc, gtot = gpu.accel(fp, ts)
+        assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc) and np.array_equal(gtot, otot)
+        assert rel_err(ga, oa) < TOL[prec]
+    gpu.set_precision(bl.BH_FP32)
+
+
+def test_large_targets_span_chunks(gpu, orc):
+    # targets of more than 64 particles (target_size 200 and a 150-fold duplicate leaf)
+    p = ic.random_box(4000, 12, 10.0)
+    p["pos"][:150] = p["pos"][0]
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    ot = orc.build(p, 4, 512)
+    oa, opp, opc, _ = ot.grouped_pass(fp, 200)
+    gpu.build_tree(p, 4, 512)
+    for prec in (bl.BH_FP64, bl.BH_FP32):
+        gpu.set_precision(prec)
+        ga, gpp, gpc, _ = gpu.accel(fp, 200)
+        assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc)
+        assert rel_err(ga, oa) < TOL[prec]
+    gpu.set_precision(bl.BH_FP32)
+
+
+def test_theta0_equals_direct_sum(gpu):
+    p = ic.random_box(1500, 1, 10.0)
+    fp = layout.force_params(0.0, 1e-6, 1.0, True)
+    gpu.build_tree(p, 4, 16)
+    gpu.set_precision(bl.BH_FP64)
+    acc = gpu.accel(fp, 0)[0]
+    gpu.set_precision(bl.BH_FP32)
+    perm = gpu.export_perm()
+    direct = gpu.direct_sum(p, perm, 1e-6)
+    assert rel_err(acc, direct) < 1e-12
+
+
+def test_direct_sum_matches_oracle(gpu, orc):
+    p = ic.random_box(3000, 9, 10.0)
+    idx = np.arange(0, 3000, 7, dtype=np.uint32)
+    assert rel_err(gpu.direct_sum(p, idx, 1e-6), orc.brute_subset(p, idx, 1e-6)) < 1e-14
+    assert rel_err(gpu.direct_sum(p[:500], None, 0.0, 2.0), orc.brute_force(p[:500], 0.0, 2.0)) < 1e-14
+
+
+@pytest.mark.parametrize("prec", [bl.BH_FP64, bl.BH_FP32])
+def test_step_parity(gpu, orc, prec):
+    p = ic.plummer(20000, 1)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    gpu.set_precision(prec)
+    out, t = gpu.step(p, fp, 16, 256, 32, 5e-4, 1e-3)
+    gpu.set_precision(bl.BH_FP32)
+    ref, _ = orc.step(p, fp, 16, 256, 32, 5e-4, 1e-3)
+    assert np.array_equal(out["id"], ref["id"]) and np.array_equal(out["mass"], ref["mass"])
+    tol = 1e-12 if prec == bl.BH_FP64 else 1e-5
+    dv = out["vel"] - ref["vel"]
+    assert np.sqrt((dv ** 2).sum() / (ref["vel"] ** 2).sum()) < tol
+    assert np.abs(out["pos"] - ref["pos"]).max() < 1e-9
+    assert t.tree_build_ms > 0 and t.traversal_ms > 0 and t.exchange_ms == 0
+
+
+def test_step_no_drift_keeps_positions(gpu):
+    p = ic.random_box(5000, 3, 5.0, vel_spread=0.2)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    out, _ = gpu.step(p, fp, 8, 256, 32, 1e-3, 0.0)
+    order = np.argsort(out["id"])
+    assert np.array_equal(out["pos"][order], p["pos"])
+
+
+def test_leapfrog_resident_state_matches_oracle(gpu, orc):
+    # the leapfrog schedule of main.cc:199-246 on resident state, 10 steps
+    p = ic.plummer(8192, 2)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    dt, steps = 1e-3, 10
+    gpu.set_precision(bl.BH_FP64)
+    gpu.state_upload(p)
+    q = p
+    sched = [(dt / 2, dt)] + [(dt, dt)] * (steps - 1) + [(dt / 2, 0.0)]
+    for k, d in sched:
+        gpu.state_step(fp, 16, 256, 32, k, d)
+        q, _ = orc.step(q, fp, 16, 256, 32, k, d)
+    out = gpu.state_download()
+    gpu.set_precision(bl.BH_FP32)
+    assert np.array_equal(out["id"], q["id"])
+    assert np.abs(out["pos"] - q["pos"]).max() < 1e-10
+    assert np.abs(out["vel"] - q["vel"]).max() < 1e-9
+
+
+def test_energy_conservation_gpu(gpu, orc):
+    # test_energy_conservation.cc: cold cube, 30 steps, |dE/E| <= 5%
+    p = ic.random_box(1000, 42, 5.0)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    dt = 1e-3
+    e0 = orc.kinetic_energy(p) + orc.potential_energy(p)
+    gpu.state_upload(p)
+    gpu.state_step(fp, 8, 256, 32, dt / 2, dt)
+    for _ in range(29):
+        gpu.state_step(fp, 8, 256, 32, dt, dt)
+    gpu.state_step(fp, 8, 256, 32, dt / 2, 0.0)
+    q = gpu.state_download()
+    e1 = orc.kinetic_energy(q) + orc.potential_energy(q)
+    assert abs((e1 - e0) / e0) <= 0.05
+
+
+def test_error_paths(gpu):
+    p = ic.random_box(1000, 1, 1.0)
+    fp = layout.force_params(0.5, 1e-6, 1.0, True)
+    bad = p.copy()
+    bad["pos"][10, 0] = np.inf
+    with pytest.raises(bl.BHError) as e:
+        gpu.step(bad, fp, 8, 256, 32, 1e-3, 1e-3)
+    assert e.value.code == 2
+    bad = p.copy()
+    bad["mass"][3] = np.nan
+    with pytest.raises(bl.BHError) as e:
+        gpu.build_tree(bad, 8, 64)
+    assert e.value.code == 2
+    with pytest.raises(bl.BHError) as e:
+        gpu.step(p, fp, 8, 0, 32, 1e-3, 1e-3)
+    assert e.value.code == 1
+    with pytest.raises(bl.BHError) as e:
+        gpu.step(p, fp, 5000, 256, 32, 1e-3, 1e-3)
+    assert e.value.code == 6
+    with pytest.raises(bl.BHError) as e:
+        gpu.step(p, layout.force_params(-1.0, 0.0, 1.0, True), 8, 256, 32, 1e-3, 1e-3)
+    assert e.value.code == 1
+    out, _ = gpu.step(p[:0], fp, 8, 256, 32, 1e-3, 1e-3)
+    assert len(out) == 0
+    # the context is still usable afterwards
+    out, _ = gpu.step(p, fp, 8, 256, 32, 1e-3, 1e-3)
+    assert np.array_equal(np.sort(out["id"]), np.arange(1000))
+
+
+def test_deterministic_replay(gpu):
+    p = ic.plummer(30000, 5)
+    fp = layout.force_params(0.5, 1e-6, 1.0, True)
+    a, _ = gpu.step(p, fp, 16, 256, 32, 1e-3, 1e-3)
+    b, _ = gpu.step(p, fp, 16, 256, 32, 1e-3, 1e-3)
+    assert golden_io.bytes_equal(a, b)
