@@ -51,6 +51,10 @@ SIGNATURES = {
     "bh_accel": ([_P, _P, _U32, _P, _P, _P, _P], _I),
     "bh_direct_sum": ([_P, _P, _U64, _P, _U64, _D, _D, _P], _I),
     "bh_last_step_stats": ([_P, _P], _I),
+    "bh_energies": ([_P, _P, _U64, _D, _P, _P], _I),
+    "bh_read_csv": ([ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_U64), ctypes.c_char_p, ctypes.c_size_t], _I),
+    "bh_free": ([_P], None),
+    "bh_write_vtk": ([ctypes.c_char_p, _P, _U64, ctypes.c_char_p, ctypes.c_size_t], _I),
 }
 
 _lib = None
@@ -98,6 +102,33 @@ def _fp(fp):
     if isinstance(fp, np.ndarray) and fp.dtype == FORCE_PARAMS:
         return fp
     raise TypeError("force params must come from bhgpu.force_params()")
+
+
+def read_csv(path):
+    """read_csv (io.cc:122-178) through the library: Particle records in file order."""
+    lib = load()
+    buf, n = _P(), _U64()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.bh_read_csv(os.fsencode(path), ctypes.byref(buf), ctypes.byref(n), err, len(err))
+    if rc != BH_OK:
+        raise BHError(rc, err.value.decode())
+    if n.value == 0:
+        return np.zeros(0, PARTICLE)
+    try:
+        raw = ctypes.string_at(buf.value, n.value * PARTICLE.itemsize)
+        return np.frombuffer(raw, dtype=PARTICLE).copy()
+    finally:
+        lib.bh_free(buf)
+
+
+def write_vtk(path, particles):
+    """write_vtk (io.cc:180-195) through the library."""
+    lib = load()
+    p = np.ascontiguousarray(particles, dtype=PARTICLE)
+    err = ctypes.create_string_buffer(512)
+    rc = lib.bh_write_vtk(os.fsencode(path), _ptr(p), len(p), err, len(err))
+    if rc != BH_OK:
+        raise BHError(rc, err.value.decode())
 
 
 class Context:
@@ -215,6 +246,14 @@ class Context:
         st = StepStats()
         self._check(self._lib.bh_last_step_stats(self._h, ctypes.byref(st)))
         return st
+
+    def energies(self, p, G=1.0, potential=True):
+        """(E_K, E_P) of host particles; E_P is the O(n^2) FP64 pair sum (None if not asked)."""
+        p = np.ascontiguousarray(p, dtype=PARTICLE)
+        ek, ep = ctypes.c_double(), ctypes.c_double()
+        self._check(self._lib.bh_energies(self._h, _ptr(p), len(p), G, ctypes.byref(ek),
+                                          ctypes.byref(ep) if potential else None))
+        return ek.value, (ep.value if potential else None)
 
     def direct_sum(self, p, idx, eps2, G=1.0):
         p = np.ascontiguousarray(p, dtype=PARTICLE)

@@ -280,6 +280,8 @@ int walk(Ctx* c, const bh_force_params* fp, uint32_t group_size, uint32_t target
   a.posm = c->posm.p;
   a.targets = c->targets.p;
   a.n_targets = c->host_misc[40];
+  a.n_nodes = c->n_nodes;
+  a.n_parts = c->n;
   a.parts = parts;
   a.acc = d_acc;
   a.n_pp = d_npp;
@@ -772,6 +774,46 @@ int bh_direct_sum(bh_ctx* h, const bh_particle* host, uint64_t n, const uint32_t
                           c->dpart.p, splits, c->dacc.p, c->stream));
   HIP_OK(c, hipMemcpyAsync(acc, c->dacc.p, 24 * m, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+int bh_energies(bh_ctx* h, const bh_particle* host, uint64_t n, double G, double* kinetic, double* potential) {
+  Ctx* c = as(h);
+  if (!c || (n && !host)) return BH_ERR_INVALID;
+  if (n > kMaxN) {
+    set_err(c, "too many particles");
+    return BH_ERR_TOO_LARGE;
+  }
+  if (kinetic) {
+    // integrate.h:27-31, summed in particle order on the host (O(n))
+    double e = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+      const double v2 = host[i].vel[0] * host[i].vel[0] + host[i].vel[1] * host[i].vel[1] + host[i].vel[2] * host[i].vel[2];
+      e += 0.5 * host[i].mass * v2;
+    }
+    *kinetic = e;
+  }
+  if (!potential) return BH_OK;
+  *potential = 0.0;
+  if (n < 2) return BH_OK;
+  HIP_OK(c, hipSetDevice(c->device));
+  std::vector<double4> src(n);
+  for (uint64_t i = 0; i < n; ++i) src[i] = make_double4(host[i].pos[0], host[i].pos[1], host[i].pos[2], host[i].mass);
+  const uint32_t blocks = static_cast<uint32_t>((n + 255) / 256);
+  uint32_t splits = 2048 / blocks;
+  if (splits < 1) splits = 1;
+  if (splits > 512) splits = 512;
+  while (splits > 1 && n / splits < 256) splits >>= 1;
+  TRY(ensure(c, c->dsrc, n, "energies"));
+  TRY(ensure(c, c->dpart, uint64_t(blocks) * splits, "energies"));
+  HIP_OK(c, hipMemcpyAsync(c->dsrc.p, src.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(c, launch_potential(c->dsrc.p, static_cast<uint32_t>(n), c->dpart.p, splits, c->stream));
+  std::vector<double> part(uint64_t(blocks) * splits);
+  HIP_OK(c, hipMemcpyAsync(part.data(), c->dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  double sum = 0.0;
+  for (double v : part) sum += v;
+  *potential = -0.5 * G * sum;
   return BH_OK;
 }
 

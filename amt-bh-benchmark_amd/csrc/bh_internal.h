@@ -57,6 +57,8 @@ struct WalkArgs {
   const double4* posm;     // x, y, z, m in Morton order
   const uint32_t* targets; // target nodes, Morton order
   uint32_t n_targets;
+  uint32_t n_nodes;        // index clamps: a node or particle index never
+  uint32_t n_parts;        // leaves its array, whatever the tree holds
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
   double* acc;             // [3n] G * acceleration (nullable)
   uint32_t* n_pp;          // [n] (nullable)
@@ -99,5 +101,7 @@ hipError_t launch_walk(const WalkArgs& a, int fp64, hipStream_t s);
 hipError_t launch_direct(const double4* src, uint32_t n, const uint32_t* tidx, uint32_t m, double eps2, double G,
                          double* partial, uint32_t splits, double* acc, hipStream_t s);
 uint32_t direct_splits(uint32_t n, uint32_t m);
+// partial[splits * ceil(n/256)]: per (slice, block) sums of m_i * sum_j m_j / r_ij
+hipError_t launch_potential(const double4* src, uint32_t n, double* partial, uint32_t splits, hipStream_t s);
 
 }  // namespace bhk

@@ -39,7 +39,7 @@ namespace bhk {
 
 namespace {
 
-constexpr int kRing = 128;  // ring capacity (entries); evaluated in batches of 64
+constexpr int kBuf = 128;  // staging capacity (entries); evaluated in batches of <= 64
 
 template <typename Real>
 struct Vec4T;
@@ -58,8 +58,9 @@ struct WaveLds {
   uint32_t fr[kLevels][kWave];  // child ranges: (first << 3) | (count - 1)
   uint32_t fr_n[kLevels];
   uint32_t fr_cur[kLevels];
-  V4 pp[kRing];       // x, y, z (relative to the target centre), m
-  V4 pc[kRing * 3];   // {x, y, z, m}, {Qxx, Qyy, Qzz, Qxy}, {Qxz, Qyz, 0, 0}
+  uint32_t own[kWave];          // lane -> owning range, via prefix max
+  V4 pp[kBuf];                  // x, y, z (relative to the target centre), m
+  V4 pc[kBuf * 3];              // {x, y, z, m}, {Qxx, Qyy, Qzz, Qxy}, {Qxz, Qyz, 0, 0}
 };
 
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -99,13 +100,28 @@ struct Acc<double> {
   __device__ inline double sz() const { return z; }
 };
 
-__device__ inline uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = __lane_id();
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(v, o);
-    if (lane >= o) v += u;
-  }
-  return v;
+// Wave64 inclusive scans on DPP (row shifts within 16-lane rows, then the
+// row broadcasts of lanes 15 and 31), no LDS round trips.
+__device__ inline uint32_t wave_incl_add(uint32_t v) {
+  int x = static_cast<int>(v);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return static_cast<uint32_t>(x);
+}
+// Inclusive max-scan of non-negative values.
+__device__ inline uint32_t wave_incl_max(uint32_t v) {
+  int x = static_cast<int>(v);
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+  return static_cast<uint32_t>(x);
 }
 
 __device__ inline uint64_t lanemask_lt() {
@@ -115,30 +131,37 @@ __device__ inline uint64_t lanemask_lt() {
 
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Largest lane j in [0, limit) with excl_j <= e (excl non-decreasing over lanes).
-__device__ inline int find_lane(uint32_t excl, uint32_t e, int limit) {
-  int j = 0;
-#pragma unroll
-  for (int step = 32; step > 0; step >>= 1) {
-    const int c = j + step;
-    const uint32_t ec = __shfl(excl, c < 64 ? c : 63);
-    if (c < limit && ec <= e) j = c;
-  }
-  return j;
+template <typename Real>
+__device__ inline typename Vec4T<Real>::T make_entry(double x, double y, double z, double m) {
+  using V4 = typename Vec4T<Real>::T;
+  V4 v;
+  v.x = static_cast<Real>(x);
+  v.y = static_cast<Real>(y);
+  v.z = static_cast<Real>(z);
+  v.w = static_cast<Real>(m);
+  return v;
 }
 
+// Padding entry: zero mass far away, contributes exactly 0 without NaNs.
+template <typename Real>
+__device__ inline typename Vec4T<Real>::T pad_entry() {
+  return make_entry<Real>(1e18, 1e18, 1e18, 0.0);
+}
+
+// P2P over this lane's C consecutive entries (grouped_ttg.h:137-153).
 template <typename Real, bool kMask>
-__device__ inline void eval_pp(const WaveLds<Real>& L, uint32_t head, uint32_t cnt, int s, int S, Real tx, Real ty, Real tz,
-                               Real eps2, Acc<Real>& acc) {
+__device__ inline void eval_pp(const typename Vec4T<Real>::T* e, int C, Real tx, Real ty, Real tz, Real eps2,
+                               Acc<Real>& acc) {
   using V4 = typename Vec4T<Real>::T;
   Real bx = 0, by = 0, bz = 0;
-  for (uint32_t j = s; j < cnt; j += S) {
-    const V4 e = L.pp[(head + j) & (kRing - 1)];
-    const Real dx = e.x - tx, dy = e.y - ty, dz = e.z - tz;
+#pragma unroll 4
+  for (int k = 0; k < C; ++k) {
+    const V4 q = e[k];
+    const Real dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
     const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
     Real inv = rsq(r2);
     if (kMask) inv = r2 > Real(0) ? inv : Real(0);
-    const Real s3 = e.w * inv * inv * inv;
+    const Real s3 = q.w * inv * inv * inv;
     bx = fmaR(s3, dx, bx);
     by = fmaR(s3, dy, by);
     bz = fmaR(s3, dz, bz);
@@ -146,16 +169,18 @@ __device__ inline void eval_pp(const WaveLds<Real>& L, uint32_t head, uint32_t c
   acc.add(bx, by, bz);
 }
 
+// Monopole + traceless quadrupole over this lane's C consecutive cells
+// (multipole.h:171-200): a = M d / r^3 - Q.d / r^5 + 2.5 (d.Q.d) d / r^7.
 template <typename Real>
-__device__ inline void eval_pc(const WaveLds<Real>& L, uint32_t head, uint32_t cnt, int s, int S, Real tx, Real ty, Real tz,
-                               Real eps2, Acc<Real>& acc) {
+__device__ inline void eval_pc(const typename Vec4T<Real>::T* e, int C, Real tx, Real ty, Real tz, Real eps2,
+                               Acc<Real>& acc) {
   using V4 = typename Vec4T<Real>::T;
   Real bx = 0, by = 0, bz = 0;
-  for (uint32_t j = s; j < cnt; j += S) {
-    const uint32_t k = (head + j) & (kRing - 1);
-    const V4 e0 = L.pc[3 * k + 0];
-    const V4 e1 = L.pc[3 * k + 1];
-    const V4 e2 = L.pc[3 * k + 2];
+#pragma unroll 2
+  for (int k = 0; k < C; ++k) {
+    const V4 e0 = e[3 * k + 0];
+    const V4 e1 = e[3 * k + 1];
+    const V4 e2 = e[3 * k + 2];
     const Real dx = e0.x - tx, dy = e0.y - ty, dz = e0.z - tz;
     const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
     const Real inv = rsq(r2);
@@ -168,23 +193,12 @@ __device__ inline void eval_pc(const WaveLds<Real>& L, uint32_t head, uint32_t c
     const Real dqd = fmaR(dx, qx, fmaR(dy, qy, dz * qz));
     const Real inv5 = inv3 * inv2;
     const Real inv7 = inv5 * inv2;
-    const Real f = fmaR(Real(2.5) * dqd, inv7, e0.w * inv3);  // M/r^3 + 2.5 dQd/r^7
+    const Real f = fmaR(Real(2.5) * dqd, inv7, e0.w * inv3);
     bx = fmaR(f, dx, fmaR(-inv5, qx, bx));
     by = fmaR(f, dy, fmaR(-inv5, qy, by));
     bz = fmaR(f, dz, fmaR(-inv5, qz, bz));
   }
   acc.add(bx, by, bz);
-}
-
-template <typename Real>
-__device__ inline typename Vec4T<Real>::T make_entry(double x, double y, double z, double m) {
-  using V4 = typename Vec4T<Real>::T;
-  V4 v;
-  v.x = static_cast<Real>(x);
-  v.y = static_cast<Real>(y);
-  v.z = static_cast<Real>(z);
-  v.w = static_cast<Real>(m);
-  return v;
 }
 
 __device__ inline double wave_min(double v) {
@@ -198,13 +212,20 @@ __device__ inline double wave_max(double v) {
 
 // One target (node tnode): the full walk for each chunk of <= 64 of its
 // particles, then reduction, output and kick/drift.
+//
+// Evaluation layout: with tc particles in the chunk, S = 64 / tc slices and
+// C = 64 / S entries per slice, a batch is B = S * C staged entries; lane
+// (p, s) evaluates entries [s*C, (s+1)*C) of the batch for particle p, so the
+// loop trip count is uniform and the LDS offsets are per-lane constants.
 template <typename Real, bool kMask>
 __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode) {
+  using V4 = typename Vec4T<Real>::T;
   const int lane = __lane_id();
   const uint32_t tb = uniform(a.begin[tnode]);
   const uint32_t T = uniform(a.count[tnode]);
   const double4 tcv = a.ctr[tnode];
   const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
+  const uint32_t last_node = a.n_nodes - 1, last_part = a.n_parts - 1;
   // FP64 bounding box of the target's particles (multipole.h:115-128)
   const double inf = __builtin_huge_val();
   double blo0 = inf, blo1 = inf, blo2 = inf, bhi0 = -inf, bhi1 = -inf, bhi2 = -inf;
@@ -230,6 +251,8 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
   for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
     const uint32_t tc = T - c0 < kWave ? T - c0 : kWave;
     const int S = kWave / static_cast<int>(tc);
+    const int C = kWave / S;
+    const uint32_t B = static_cast<uint32_t>(S * C);
     const bool active = lane < static_cast<int>(tc) * S;
     const int p = lane % static_cast<int>(tc), s = lane / static_cast<int>(tc);
     const uint32_t pidx = tb + c0 + p;
@@ -240,9 +263,46 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
       ty = static_cast<Real>(q.y - tcy);
       tz = static_cast<Real>(q.z - tcz);
     }
+    const V4* my_pp = &L.pp[s * C];
+    const V4* my_pc = &L.pc[3 * s * C];
     Acc<Real> acc;
-    uint32_t pp_head = 0, pp_cnt = 0, pc_head = 0, pc_cnt = 0;
+    uint32_t pp_cnt = 0, pc_cnt = 0;
     uint32_t n_pp = 0, n_pc = 0;
+
+    // Evaluate the first B staged entries, then move the rest (< 64: at most
+    // B - 1 + 64 were staged) to the front; callers repeat while >= B remain.
+    auto flush_pp = [&]() {
+      __builtin_amdgcn_wave_barrier();
+      if (active) eval_pp<Real, kMask>(my_pp, C, tx, ty, tz, eps2, acc);
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t rest = pp_cnt - B;
+      V4 v;
+      if (static_cast<uint32_t>(lane) < rest) v = L.pp[B + lane];
+      __builtin_amdgcn_wave_barrier();
+      if (static_cast<uint32_t>(lane) < rest) L.pp[lane] = v;
+      pp_cnt = rest;
+      __builtin_amdgcn_wave_barrier();
+    };
+    auto flush_pc = [&]() {
+      __builtin_amdgcn_wave_barrier();
+      if (active) eval_pc<Real>(my_pc, C, tx, ty, tz, eps2, acc);
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t rest = pc_cnt - B;
+      V4 v0, v1, v2;
+      if (static_cast<uint32_t>(lane) < rest) {
+        v0 = L.pc[3 * (B + lane) + 0];
+        v1 = L.pc[3 * (B + lane) + 1];
+        v2 = L.pc[3 * (B + lane) + 2];
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (static_cast<uint32_t>(lane) < rest) {
+        L.pc[3 * lane + 0] = v0;
+        L.pc[3 * lane + 1] = v1;
+        L.pc[3 * lane + 2] = v2;
+      }
+      pc_cnt = rest;
+      __builtin_amdgcn_wave_barrier();
+    };
 
     if (lane == 0) {
       L.fr[0][0] = 0u;  // node 0, one node
@@ -258,29 +318,35 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
         --depth;
         continue;
       }
-      // pop whole child ranges, up to 64 nodes
+      // Pop whole child ranges, up to 64 nodes. Ranges of a frame are
+      // ascending and disjoint, so first - excl is non-decreasing over the
+      // ranges and a prefix max of it, placed at each range's first slot,
+      // yields every slot's node: node = (first_j - excl_j) + slot.
       const uint32_t rr = cur + lane;
       const bool rv = rr < nr;
       const uint32_t packed = rv ? L.fr[depth][rr] : 0u;
       const uint32_t rcnt = rv ? (packed & 7u) + 1u : 0u;
       const uint32_t rfirst = packed >> 3;
-      const uint32_t incl = wave_incl_scan(rcnt);
-      const uint64_t tm = __ballot(rv && incl <= kWave);
-      const int k = __popcll(tm);
-      const uint32_t total = uniform(__shfl(incl, k - 1));
+      const uint32_t incl = wave_incl_add(rcnt);
+      const bool taken = rv && incl <= kWave;
+      const int k = __popcll(__ballot(taken));
+      const uint32_t total = uniform(__builtin_amdgcn_readlane(incl, k - 1));
       const uint32_t excl = incl - rcnt;
+      L.own[lane] = 0u;
       __builtin_amdgcn_wave_barrier();
+      if (taken) L.own[excl] = rfirst - excl;
       if (lane == 0) L.fr_cur[depth] = cur + k;
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t node = wave_incl_max(L.own[lane]) + static_cast<uint32_t>(lane);
       const bool nv = static_cast<uint32_t>(lane) < total;
-      const int j = find_lane(excl, static_cast<uint32_t>(lane), k);
-      const uint32_t node = __shfl(rfirst, j) + (static_cast<uint32_t>(lane) - __shfl(excl, j));
       // MAC in FP64, reference operation order
       bool accept = false, isleaf = false;
       uint32_t lfirst = 0, lcnt = 0;
       double4 mac = make_double4(0, 0, 0, 0);
+      const uint32_t nd = node < last_node ? node : last_node;
       if (nv) {
-        mac = a.mac[node];
-        const uint2 lk = a.link[node];
+        mac = a.mac[nd];
+        const uint2 lk = a.link[nd];
         const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
         const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
         const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
@@ -311,14 +377,14 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
         const uint32_t pos = __popcll(cm & lt);
         if (a.use_quad) {
           if (cellf) {
-            const uint32_t slot = (pc_head + pc_cnt + pos) & (kRing - 1);
+            const uint32_t slot = pc_cnt + pos;
             if constexpr (sizeof(Real) == 4) {
-              const float4 q0 = a.q32[2 * node + 0], q1 = a.q32[2 * node + 1];
+              const float4 q0 = a.q32[2 * nd + 0], q1 = a.q32[2 * nd + 1];
               L.pc[3 * slot + 0] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, q0.x);
               L.pc[3 * slot + 1] = make_float4(q0.y, q0.z, q0.w, q1.x);
               L.pc[3 * slot + 2] = make_float4(q1.y, q1.z, 0.f, 0.f);
             } else {
-              const double* q = &a.q64[8 * node];
+              const double* q = &a.q64[8 * nd];
               L.pc[3 * slot + 0] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, q[0]);
               L.pc[3 * slot + 1] = make_double4(q[1], q[2], q[3], q[4]);
               L.pc[3 * slot + 2] = make_double4(q[5], q[6], 0.0, 0.0);
@@ -326,63 +392,69 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
           }
           pc_cnt += nc;
           n_pc += nc;
-          __builtin_amdgcn_wave_barrier();
-          if (pc_cnt >= kWave) {
-            if (active) eval_pc<Real>(L, pc_head, kWave, s, S, tx, ty, tz, eps2, acc);
-            pc_head += kWave;
-            pc_cnt -= kWave;
-          }
+          while (pc_cnt >= B) flush_pc();
         } else {
           if (cellf) {
-            const uint32_t slot = (pp_head + pp_cnt + pos) & (kRing - 1);
-            const double m = sizeof(Real) == 4 ? static_cast<double>(a.q32[2 * node].x) : a.q64[8 * node];
-            L.pp[slot] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, m);
+            const double m = sizeof(Real) == 4 ? static_cast<double>(a.q32[2 * nd].x) : a.q64[8 * nd];
+            L.pp[pp_cnt + pos] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, m);
           }
           pp_cnt += nc;
           n_pp += nc;
-          __builtin_amdgcn_wave_barrier();
-          if (pp_cnt >= kWave) {
-            if (active) eval_pp<Real, kMask>(L, pp_head, kWave, s, S, tx, ty, tz, eps2, acc);
-            pp_head += kWave;
-            pp_cnt -= kWave;
-          }
+          while (pp_cnt >= B) flush_pp();
         }
       }
-      // particles of opened leaves
+      // particles of opened leaves; particle ranges of one iteration's
+      // leaves are ascending and disjoint, mapped to slots like the nodes
       const uint64_t lm = __ballot(leaff);
       if (lm) {
         const uint32_t lc = leaff ? lcnt : 0u;
-        const uint32_t lincl = wave_incl_scan(lc);
+        const uint32_t lincl = wave_incl_add(lc);
         const uint32_t lexcl = lincl - lc;
-        const uint32_t ltot = uniform(__shfl(lincl, 63));
+        const uint32_t ltot = uniform(__builtin_amdgcn_readlane(lincl, 63));
+        const uint32_t lval = lfirst - lexcl;
         for (uint32_t b = 0; b < ltot; b += kWave) {
+          // the leaf holding slot b: the last leaf lane starting at or before it
+          const uint64_t before = __ballot(leaff && lexcl <= b);
+          const int j0 = 63 - __clzll(static_cast<long long>(before));
+          const uint32_t v0 = uniform(__builtin_amdgcn_readlane(lval, j0));
+          L.own[lane] = v0;
+          __builtin_amdgcn_wave_barrier();
+          if (leaff && lexcl > b && lexcl < b + kWave) L.own[lexcl - b] = lval;
+          __builtin_amdgcn_wave_barrier();
           const uint32_t e = b + lane;
-          const int jl = find_lane(lexcl, e, kWave);
-          const uint32_t q = __shfl(lfirst, jl) + (e - __shfl(lexcl, jl));
+          uint32_t q = wave_incl_max(L.own[lane]) + e;
+          q = q < last_part ? q : last_part;
           const uint32_t chunk = ltot - b < kWave ? ltot - b : kWave;
           if (e < ltot) {
             const double4 sp = a.posm[q];
-            L.pp[(pp_head + pp_cnt + lane) & (kRing - 1)] = make_entry<Real>(sp.x - tcx, sp.y - tcy, sp.z - tcz, sp.w);
+            L.pp[pp_cnt + lane] = make_entry<Real>(sp.x - tcx, sp.y - tcy, sp.z - tcz, sp.w);
           }
           pp_cnt += chunk;
           n_pp += chunk;
-          __builtin_amdgcn_wave_barrier();
-          if (pp_cnt >= kWave) {
-            if (active) eval_pp<Real, kMask>(L, pp_head, kWave, s, S, tx, ty, tz, eps2, acc);
-            pp_head += kWave;
-            pp_cnt -= kWave;
-          }
+          while (pp_cnt >= B) flush_pp();
         }
       }
       __builtin_amdgcn_wave_barrier();
       if (om) ++depth;
     }
-    if (active) {
-      if (pp_cnt) eval_pp<Real, kMask>(L, pp_head, pp_cnt, s, S, tx, ty, tz, eps2, acc);
-      if (pc_cnt) eval_pc<Real>(L, pc_head, pc_cnt, s, S, tx, ty, tz, eps2, acc);
+    // final partial batches (fewer than B entries), padded to B
+    if (pp_cnt) {
+      if (static_cast<uint32_t>(lane) >= pp_cnt && static_cast<uint32_t>(lane) < B) L.pp[lane] = pad_entry<Real>();
+      pp_cnt = B;
+      flush_pp();
+    }
+    if (pc_cnt) {
+      if (static_cast<uint32_t>(lane) >= pc_cnt && static_cast<uint32_t>(lane) < B) {
+        const V4 z = make_entry<Real>(0.0, 0.0, 0.0, 0.0);
+        L.pc[3 * lane + 0] = pad_entry<Real>();
+        L.pc[3 * lane + 1] = z;
+        L.pc[3 * lane + 2] = z;
+      }
+      pc_cnt = B;
+      flush_pc();
     }
     __builtin_amdgcn_wave_barrier();
-    // combine slices in FP64 (the P2C ring is free now)
+    // combine slices in FP64 (the P2C buffer is free now)
     double* red = reinterpret_cast<double*>(L.pc);
     if (active) {
       red[3 * lane + 0] = acc.sx();
@@ -504,6 +576,40 @@ __global__ __launch_bounds__(256) void k_direct_reduce(const double* __restrict_
   acc[3 * uint64_t(k) + 2] = G * a2;
 }
 
+// potential_energy (integrate.h:35-43): each target i sums m_j / r_ij over a
+// slice of the sources j != i; the per-(block, slice) partials of
+// m_i * sum are reduced in a fixed order, and E_P = -G/2 * total.
+__global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ src, uint32_t n,
+                                                   double* __restrict__ partial, uint32_t splits) {
+  __shared__ double4 s_src[256];
+  __shared__ double s_red[4];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const uint32_t lo = static_cast<uint32_t>(uint64_t(n) * split / splits);
+  const uint32_t hi = static_cast<uint32_t>(uint64_t(n) * (split + 1) / splits);
+  const bool ok = i < n;
+  double4 t = make_double4(0, 0, 0, 0);
+  if (ok) t = src[i];
+  double e = 0.0;
+  for (uint32_t b = lo; b < hi; b += 256) {
+    __syncthreads();
+    if (b + threadIdx.x < hi) s_src[threadIdx.x] = src[b + threadIdx.x];
+    __syncthreads();
+    const uint32_t cnt = hi - b < 256 ? hi - b : 256;
+    for (uint32_t jj = 0; jj < cnt; ++jj) {
+      if (b + jj == i) continue;
+      const double4 q = s_src[jj];
+      const double dx = t.x - q.x, dy = t.y - q.y, dz = t.z - q.z;
+      e += q.w / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  }
+  double v = ok ? t.w * e : 0.0;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[uint64_t(split) * gridDim.x + blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
 template <typename Real, int kWaves>
 hipError_t launch_walk_t(const WalkArgs& a, hipStream_t s) {
   uint32_t blocks = (a.n_targets + kWaves - 1) / kWaves;
@@ -520,6 +626,12 @@ hipError_t launch_walk_t(const WalkArgs& a, hipStream_t s) {
 hipError_t launch_walk(const WalkArgs& a, int fp64, hipStream_t s) {
   if (a.n_targets == 0) return hipSuccess;
   return fp64 ? launch_walk_t<double, 2>(a, s) : launch_walk_t<float, 2>(a, s);
+}
+
+hipError_t launch_potential(const double4* src, uint32_t n, double* partial, uint32_t splits, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_potential, dim3((n + 255) / 256, splits), dim3(256), 0, s, src, n, partial, splits);
+  return hipGetLastError();
 }
 
 uint32_t direct_splits(uint32_t n, uint32_t m) {

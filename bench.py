@@ -104,7 +104,20 @@ def main():
     ap.add_argument("--error-subset", type=int, default=4096)
     ap.add_argument("--cpu-stride", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (torch) three times, to calibrate HBM byte counters")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(args.copy_calibration * (1 << 18), dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4, "repeats": 3}))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this build has no multi-GPU path; "

@@ -16,6 +16,7 @@
 #ifndef BH_GPU_H
 #define BH_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -164,6 +165,21 @@ int bh_last_step_stats(bh_ctx* ctx, bh_step_stats* stats);
  * host particles[0..n), FP64 on the device; acc[3*m]. idx NULL = all. */
 int bh_direct_sum(bh_ctx* ctx, const bh_particle* host, uint64_t n, const uint32_t* idx, uint64_t m, double eps2, double G,
                   double* acc);
+
+/* kinetic_energy and potential_energy (integrate.h:27-43) of host particles
+ * [0..n), FP64 on the device: E_K = sum m v^2 / 2; E_P = -G sum_{i<j} m_i m_j / r_ij
+ * (unsoftened, O(n^2)). Either output may be NULL (E_P is then not computed). */
+int bh_energies(bh_ctx* ctx, const bh_particle* host, uint64_t n, double G, double* kinetic, double* potential);
+
+/* read_csv (io.h:10-14, io.cc:94-178): particles "id,mass,x,y,z,vx,vy,vz", one
+ * per line, optional header line. *out is allocated with malloc (free it
+ * with bh_free); err (nullable) receives the message of a failure, which
+ * names the first malformed line. Host only. */
+int bh_read_csv(const char* path, bh_particle** out, uint64_t* n, char* err, size_t err_len);
+void bh_free(void* p);
+/* write_vtk (io.h:16-18, io.cc:180-195): legacy VTK ASCII PolyData with
+ * points, a mass scalar and a velocity vector. Host only. */
+int bh_write_vtk(const char* path, const bh_particle* particles, uint64_t n, char* err, size_t err_len);
 
 #ifdef __cplusplus
 }
