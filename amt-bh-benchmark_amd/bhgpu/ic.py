@@ -19,6 +19,16 @@ position angles (slots 64, 65); speed by Aarseth-Henon-Wielen rejection
 (1974): q, y uniform, accept when 0.1*y < q^2 (1-q^2)^3.5 (slot pairs
 128+2k, 129+2k), v = q * sqrt(2) * (1 + r^2)^(-1/4); isotropic velocity
 angles (slots 1024, 1025). Masses 1/N.
+
+Hernquist (G = M = a = 1, mixed-N ensemble): radius by inverse CDF of
+M(<r) = r^2 / (1 + r)^2, r = sqrt(u) / (1 - sqrt(u)), redrawn until r < 20 a
+(slots 0..63); isotropic position angles (slots 64, 65); speed from the
+isotropic ergodic distribution function (Hernquist 1990, ApJ 356, 359,
+eq. 17) by rejection in s = v / v_esc: s, y uniform, accept when
+y * B(r) < s^2 f(psi (1 - s^2)) with psi = 1 / (1 + r) and B(r) 1.25 times
+the largest value of s^2 f(psi (1 - s^2)) over a 257-point grid of s (slot
+pairs 128+2k, 129+2k); isotropic velocity angles (slots 1024, 1025).
+Masses 1/N.
 """
 import numpy as np
 
@@ -127,4 +137,61 @@ def random_box(n, seed, spread, center=(0.0, 0.0, 0.0), vel_spread=0.0):
     p["pos"] = np.stack([center[a] + spread * (2.0 * uniform(st, 1 + a) - 1.0) for a in range(3)], axis=1)
     if vel_spread:
         p["vel"] = np.stack([vel_spread * (2.0 * uniform(st, 4 + a) - 1.0) for a in range(3)], axis=1)
+    return p
+
+
+def hernquist_df(E):
+    """Isotropic distribution function of the Hernquist model, G = M = a = 1
+    (Hernquist 1990, eq. 17), for relative binding energies 0 <= E < 1."""
+    E = np.asarray(E, dtype=np.float64)
+    q = np.sqrt(np.clip(E, 0.0, 1.0 - 1e-16))
+    q2 = q * q
+    core = 3.0 * np.arcsin(q) + q * np.sqrt(1.0 - q2) * (1.0 - 2.0 * q2) * (8.0 * q2 * q2 - 8.0 * q2 - 3.0)
+    return np.maximum(core, 0.0) / ((1.0 - q2) ** 2.5 * 8.0 * np.sqrt(2.0) * np.pi ** 3)
+
+
+def hernquist(n, seed, rmax=20.0):
+    """Hernquist sphere, G = M = a = 1, truncated at r < rmax, isotropic
+    ergodic velocities (mixed-N ensemble)."""
+    idx = np.arange(n, dtype=np.uint64)
+    st = stream_state(seed, idx)
+    r = np.full(n, np.nan)
+    todo = np.arange(n)
+    for k in range(64):
+        if todo.size == 0:
+            break
+        su = np.sqrt(uniform(st[todo], k))
+        with np.errstate(divide="ignore"):
+            rr = su / (1.0 - su)
+        ok = rr < rmax
+        r[todo[ok]] = rr[ok]
+        todo = todo[~ok]
+    if todo.size:
+        raise RuntimeError("hernquist: radius rejection did not converge")
+    pos = _isotropic(st, 64) * r[:, None]
+    psi = 1.0 / (1.0 + r)
+    grid = np.linspace(0.0, 1.0, 257)
+    bound = np.empty(n)
+    for lo in range(0, n, 1 << 16):
+        ps = psi[lo:lo + (1 << 16), None]
+        bound[lo:lo + (1 << 16)] = 1.25 * (grid * grid * hernquist_df(ps * (1.0 - grid * grid))).max(axis=1)
+    s_v = np.full(n, np.nan)
+    todo = np.arange(n)
+    for k in range(400):
+        if todo.size == 0:
+            break
+        x = uniform(st[todo], 128 + 2 * k)
+        y = uniform(st[todo], 129 + 2 * k)
+        ok = y * bound[todo] < x * x * hernquist_df(psi[todo] * (1.0 - x * x))
+        s_v[todo[ok]] = x[ok]
+        todo = todo[~ok]
+    if todo.size:
+        raise RuntimeError("hernquist: velocity rejection did not converge")
+    speed = s_v * np.sqrt(2.0 * psi)
+    vel = _isotropic(st, 1024) * speed[:, None]
+    p = np.zeros(n, PARTICLE)
+    p["id"] = np.arange(n, dtype=np.uint32)
+    p["mass"] = 1.0 / n
+    p["pos"] = pos
+    p["vel"] = vel
     return p

@@ -37,6 +37,24 @@ FORCE_PARAMS = np.dtype(
 assert FORCE_PARAMS.itemsize == 32
 
 
+SYSTEM = np.dtype(
+    [("n", "<u8"), ("fp", FORCE_PARAMS), ("max_leaf", "<u4"), ("group_size", "<u4"), ("target_size", "<u4"),
+     ("pad", "<u4"), ("kick_dt", "<f8"), ("drift_dt", "<f8")],
+    align=True,
+)
+assert SYSTEM.itemsize == 72
+
+
+def system(n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
+    """One bh_system record."""
+    s = np.zeros(1, SYSTEM)
+    s["n"] = n
+    s["fp"] = fp[0] if isinstance(fp, np.ndarray) else fp
+    s["max_leaf"], s["group_size"], s["target_size"] = max_leaf, group_size, target_size
+    s["kick_dt"], s["drift_dt"] = kick_dt, drift_dt
+    return s
+
+
 def force_params(theta=0.5, eps2=0.0, G=1.0, use_quadrupole=True):
     """ForceParams with the reference's defaults (traversal_ttg.h:136-141)."""
     fp = np.zeros(1, FORCE_PARAMS)

@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .layout import FORCE_PARAMS, PARTICLE, WALK_NODE
+from .layout import FORCE_PARAMS, PARTICLE, SYSTEM, WALK_NODE
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libbhgpu.so")
@@ -41,6 +41,8 @@ SIGNATURES = {
     "bh_max_particles": ([], _U64),
     "bh_step": ([_P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_step_device": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
+    "bh_step_batch": ([_P, _P, _P, _P, _U32, _P], _I),
+    "bh_step_batch_host": ([_P, _P, _P, _U32, _P], _I),
     "bh_state_upload": ([_P, _P, _U64], _I),
     "bh_state_step": ([_P, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_state_download": ([_P, _P, _U64], _I),
@@ -175,6 +177,23 @@ class Context:
         t = StepTiming()
         self._check(self._lib.bh_step_device(self._h, d_in, d_out, n, _ptr(_fp(fp)), max_leaf, group_size, target_size,
                                              kick_dt, drift_dt, ctypes.byref(t)))
+        return t
+
+    def step_batch(self, p, systems):
+        """bh_step_batch_host: systems (SYSTEM records) own consecutive ranges of
+        p; returns (particles, each system in its own Morton order, StepTiming)."""
+        systems = np.ascontiguousarray(systems, dtype=SYSTEM)
+        out = np.array(p, dtype=PARTICLE, copy=True)
+        if int(systems["n"].sum()) != len(out):
+            raise ValueError("system sizes do not add up to the particle count")
+        t = StepTiming()
+        self._check(self._lib.bh_step_batch_host(self._h, _ptr(out), _ptr(systems), len(systems), ctypes.byref(t)))
+        return out, t
+
+    def step_batch_device(self, d_in, d_out, systems):
+        systems = np.ascontiguousarray(systems, dtype=SYSTEM)
+        t = StepTiming()
+        self._check(self._lib.bh_step_batch(self._h, d_in, d_out, _ptr(systems), len(systems), ctypes.byref(t)))
         return t
 
     def state_upload(self, p):

@@ -66,11 +66,30 @@ struct Ctx {
   DevBuf<double> dpart, dacc;
 
   uint32_t*  host_misc = nullptr;   // pinned
+  // system table of the current batch: device copy and pinned staging
+  DevBuf<uint8_t> tables;           // SysDev[S] | uint4 chunks[C] | uint32 chunk_first[S + 1] | uint32 sid hist[2][256]
+  uint8_t* host_tables = nullptr;
+  size_t host_tables_cap = 0;
+  DevBuf<uint16_t> sid;
+  uint32_t n_sys = 0, n_chunks = 0, mlx = 0;
+  int sid_passes = 0;
+  bool any_unsoftened = false;
+  size_t off_chunks = 0, off_chunk_first = 0, off_sid_hist = 0;
   // last built tree
   bool has_tree = false;
   uint32_t n = 0, n_nodes = 0, n_leaves = 0;
   uint32_t max_leaf = 0, group_size = 0;
   bh_particle* sorted = nullptr;    // device: Morton-ordered particles of the last tree
+  uint64_t* keys_sorted = nullptr;  // device: sorted keys of the last tree
+  uint32_t* perm_sorted = nullptr;  // device: sorted position -> input index
+};
+
+// One system of a batch as the host sees it.
+struct HostSys {
+  uint64_t n;
+  bh_force_params fp;
+  uint32_t max_leaf, group_size, target_size;
+  double kick_dt, drift_dt;
 };
 
 bool set_err(Ctx* c, const char* fmt, ...) {
@@ -172,20 +191,117 @@ TreeDev tree_view(Ctx* c) {
   return t;
 }
 
-// build_flat_tree (flat_build_ttg.h:276-281) of d_in[0..n) into the context;
-// the Morton-ordered particles go to d_out.
-int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n, uint32_t max_leaf) {
+// The batch's system table: per-system parameters, chunks of <= kChunk
+// particles for the per-system bounds, and the system-id digit counts of the
+// sort's last passes. Staged in pinned memory, uploaded with one copy.
+int prepare_systems(Ctx* c, const HostSys* sys, uint32_t S, uint64_t* n_total) {
+  if (S == 0 || S > kMaxSystems) {
+    set_err(c, "a batch holds 1..%u systems, %u given", kMaxSystems, S);
+    return BH_ERR_INVALID;
+  }
+  // Empty systems take no part: the device table lists the others, in
+  // order, and a particle's system id indexes that list.
+  uint64_t n = 0, chunks = 0;
+  uint32_t mlx = 0, live = 0;
+  bool unsoftened = false;
+  for (uint32_t k = 0; k < S; ++k) {
+    TRY(check_args(c, sys[k].n, &sys[k].fp, sys[k].max_leaf, sys[k].group_size));
+    if (sys[k].n == 0) continue;
+    ++live;
+    n += sys[k].n;
+    chunks += (sys[k].n + kChunk - 1) / kChunk;
+    mlx = std::max(mlx, sys[k].max_leaf);
+    unsoftened |= static_cast<float>(sys[k].fp.eps2) == 0.0f;  // zero in the FP32 walk too
+  }
+  if (n > kMaxN) {
+    set_err(c, "batch of %llu particles exceeds the supported %llu", (unsigned long long)n, (unsigned long long)kMaxN);
+    return BH_ERR_TOO_LARGE;
+  }
+  *n_total = n;
+  if (n == 0) return BH_OK;
+  const uint32_t S_all = S;
+  S = live;
+  const size_t off_chunks = sizeof(SysDev) * S;
+  const size_t off_first = off_chunks + sizeof(uint4) * chunks;
+  const size_t off_hist = (off_first + 4 * (S + 1) + 15) / 16 * 16;
+  const size_t bytes = off_hist + 2 * 256 * 4;
+  if (c->host_tables_cap < bytes) {
+    if (c->host_tables) (void)hipHostFree(c->host_tables);
+    c->host_tables = nullptr;
+    c->host_tables_cap = 0;
+    const size_t want = bytes + bytes / 2;
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->host_tables), want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      set_err(c, "pinned host allocation of %zu bytes failed", want);
+      return BH_ERR_OOM;
+    }
+    c->host_tables_cap = want;
+  }
+  std::memset(c->host_tables, 0, bytes);
+  SysDev* sd = reinterpret_cast<SysDev*>(c->host_tables);
+  uint4* ch = reinterpret_cast<uint4*>(c->host_tables + off_chunks);
+  uint32_t* first = reinterpret_cast<uint32_t*>(c->host_tables + off_first);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(c->host_tables + off_hist);
+  uint64_t off = 0, ci = 0;
+  for (uint32_t k = 0, src = 0; k < S; ++k, ++src) {
+    while (sys[src].n == 0) ++src;
+    const HostSys& h = sys[src];
+    sd[k].theta2 = h.fp.theta * h.fp.theta;
+    sd[k].eps2 = h.fp.eps2;
+    sd[k].G = h.fp.G;
+    sd[k].kick_dt = h.kick_dt;
+    sd[k].drift_dt = h.drift_dt;
+    sd[k].off = static_cast<uint32_t>(off);
+    sd[k].n = static_cast<uint32_t>(h.n);
+    sd[k].max_leaf = h.max_leaf;
+    sd[k].tsm = std::min(h.target_size, h.group_size);
+    sd[k].use_quad = h.fp.use_quadrupole != 0;
+    first[k] = static_cast<uint32_t>(ci);
+    for (uint64_t b = 0; b < h.n; b += kChunk)
+      ch[ci++] = make_uint4(k, static_cast<uint32_t>(off + b), static_cast<uint32_t>(off + std::min<uint64_t>(h.n, b + kChunk)), 0);
+    hist[k & 255] += static_cast<uint32_t>(h.n);
+    hist[256 + (k >> 8)] += static_cast<uint32_t>(h.n);
+    off += h.n;
+  }
+  first[S] = static_cast<uint32_t>(ci);
+  TRY(ensure(c, c->tables, bytes, "system table"));
+  TRY(ensure(c, c->sid, n, "system ids"));
+  HIP_OK(c, hipMemcpyAsync(c->tables.p, c->host_tables, bytes, hipMemcpyHostToDevice, c->stream));
+  c->n_sys = S;
+  c->n_chunks = static_cast<uint32_t>(chunks);
+  c->mlx = mlx;
+  c->sid_passes = S == 1 ? 0 : (S <= 256 ? 1 : 2);
+  c->any_unsoftened = unsoftened;
+  c->off_chunks = off_chunks;
+  c->off_chunk_first = off_first;
+  c->off_sid_hist = off_hist;
+  // single-system queries (export, accel) read the first system's shape
+  for (uint32_t k = 0; k < S_all; ++k) {
+    if (sys[k].n == 0) continue;
+    c->max_leaf = sys[k].max_leaf;
+    c->group_size = sys[k].group_size;
+    break;
+  }
+  return BH_OK;
+}
+
+const SysDev* sys_dev(Ctx* c) { return reinterpret_cast<const SysDev*>(c->tables.p); }
+
+// build_flat_tree (flat_build_ttg.h:276-281) of every system of the prepared
+// batch, d_in[0..n) into the context; the Morton-ordered particles (each
+// system in its own range) go to d_out.
+int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n) {
   hipStream_t s = c->stream;
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   TRY(ensure(c, c->keys_a, n, "keys"));
   TRY(ensure(c, c->keys_b, n, "keys"));
   TRY(ensure(c, c->vals_a, n, "indices"));
   TRY(ensure(c, c->vals_b, n, "indices"));
-  TRY(ensure(c, c->hist, 8 * 256, "histograms"));
-  TRY(ensure(c, c->dbase, 8 * 256, "digit offsets"));
+  TRY(ensure(c, c->hist, 10 * 256, "histograms"));
+  TRY(ensure(c, c->dbase, 10 * 256, "digit offsets"));
   TRY(ensure(c, c->lookback, radix_lookback_words(n), "look-back"));
-  TRY(ensure(c, c->lohi, 8, "bounds"));
-  TRY(ensure(c, c->partial, 6 * 1024, "bounds"));
+  TRY(ensure(c, c->lohi, 6ull * c->n_sys, "bounds"));
+  TRY(ensure(c, c->partial, 6ull * c->n_chunks, "bounds"));
   TRY(ensure(c, c->misc, 64, "flags"));
   TRY(ensure(c, c->posm, n, "walk copy"));
   TRY(ensure(c, c->L, n, "tree build"));
@@ -194,17 +310,24 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n, uint3
   TRY(ensure(c, c->tprefix, uint64_t(tiles) * kTreeCols, "tree build"));
   TRY(ensure(c, c->level_off, kLevels + 2, "tree levels"));
 
+  const uint4* chunks = reinterpret_cast<const uint4*>(c->tables.p + c->off_chunks);
+  const uint32_t* chunk_first = reinterpret_cast<const uint32_t*>(c->tables.p + c->off_chunk_first);
   HIP_OK(c, launch_fill_u32(c->misc.p, 0u, 64, s));
   HIP_OK(c, launch_fill_u32(c->hist.p, 0u, 8 * 256, s));
+  if (c->sid_passes)
+    HIP_OK(c, hipMemcpyAsync(c->hist.p + 8 * 256, c->host_tables + c->off_sid_hist, 2 * 256 * 4, hipMemcpyHostToDevice, s));
   HIP_OK(c, launch_fill_u32(c->lookback.p, 0u, radix_lookback_words(n), s));
-  HIP_OK(c, launch_bounds(d_in, n, c->partial.p, c->misc.p, c->lohi.p, s));
-  HIP_OK(c, launch_keys_hist(d_in, n, c->lohi.p, c->keys_a.p, c->hist.p, s));
+  HIP_OK(c, launch_fill_sid(chunks, c->n_chunks, c->sid.p, s));
+  HIP_OK(c, launch_bounds(d_in, chunks, c->n_chunks, chunk_first, c->n_sys, c->partial.p, c->misc.p, c->lohi.p, s));
+  HIP_OK(c, launch_keys_hist(d_in, n, c->sid.p, c->lohi.p, c->keys_a.p, c->hist.p, s));
   const uint64_t lb_words = radix_lookback_words(n);
-  HIP_OK(c, launch_radix_sort(c->keys_a.p, c->vals_a.p, c->keys_b.p, c->vals_b.p, n, c->hist.p, c->dbase.p, c->lookback.p,
-                              c->lookback.p + (lb_words - 8), s));
-  HIP_OK(c, launch_gather(d_in, d_out, c->vals_a.p, c->posm.p, n, s));
+  uint64_t* keys_sorted = nullptr;
+  uint32_t* perm = nullptr;
+  HIP_OK(c, launch_radix_sort(c->keys_a.p, c->vals_a.p, c->keys_b.p, c->vals_b.p, n, c->sid_passes, c->sid.p, c->hist.p,
+                              c->dbase.p, c->lookback.p, c->lookback.p + (lb_words - 10), &keys_sorted, &perm, s));
+  HIP_OK(c, launch_gather(d_in, d_out, perm, c->posm.p, n, s));
   HIP_OK(c, hipEventRecord(c->ev[1], s));
-  HIP_OK(c, launch_tree_count(c->keys_a.p, n, max_leaf, c->L.p, c->H.p, c->tcounts.p, s));
+  HIP_OK(c, launch_tree_count(keys_sorted, n, c->sid.p, sys_dev(c), c->mlx, c->L.p, c->H.p, c->tcounts.p, s));
   HIP_OK(c, launch_tree_scan(c->tcounts.p, tiles, c->tprefix.p, c->misc.p + 8, s));
   HIP_OK(c, hipMemcpyAsync(c->host_misc, c->misc.p, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(c, hipStreamSynchronize(s));
@@ -241,29 +364,29 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n, uint3
   TRY(ensure(c, c->leaf_first, leaves, "leaves"));
   const TreeDev t = tree_view(c);
   HIP_OK(c, launch_tree_emit(c->L.p, c->H.p, n, c->tprefix.p, c->misc.p + 8, t, s));
-  HIP_OK(c, launch_tree_finish(c->keys_a.p, c->posm.p, n, c->lohi.p, c->misc.p + 8, m, leaves, t, s));
+  HIP_OK(c, launch_tree_finish(keys_sorted, c->posm.p, n, c->sid.p, c->lohi.p, m, leaves, t, s));
   HIP_OK(c, hipEventRecord(c->ev[2], s));
   c->n = n;
   c->n_nodes = m;
   c->n_leaves = leaves;
-  c->max_leaf = max_leaf;
   c->sorted = d_out;
+  c->keys_sorted = keys_sorted;
+  c->perm_sorted = perm;
   c->has_tree = true;
   return BH_OK;
 }
 
-// run_grouped_pass (grouped_ttg.h:297-323) over the context's tree.
-int walk(Ctx* c, const bh_force_params* fp, uint32_t group_size, uint32_t target_size, bh_particle* parts, double kick_dt,
-         double drift_dt, double* d_acc, uint32_t* d_npp, uint32_t* d_npc) {
+// run_grouped_pass (grouped_ttg.h:297-323) over every system of the
+// context's tree, with each system's parameters from the system table.
+int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d_npc) {
   hipStream_t s = c->stream;
   const uint32_t n = c->n;
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   TRY(ensure(c, c->mark, n, "targets"));
   TRY(ensure(c, c->targets, n, "targets"));
   TRY(ensure(c, c->totals, 2, "totals"));
-  const uint32_t tsm = std::min(target_size, group_size);
-  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, tsm, c->mark.p, c->tcounts.p, c->tprefix.p, c->targets.p,
-                           c->misc.p + 40, s));
+  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), c->mark.p, c->tcounts.p, c->tprefix.p,
+                           c->targets.p, c->misc.p + 40, s));
   HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
   HIP_OK(c, hipEventRecord(c->ev[3], s));
   HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -279,6 +402,8 @@ int walk(Ctx* c, const bh_force_params* fp, uint32_t group_size, uint32_t target
   a.count = c->count.p;
   a.posm = c->posm.p;
   a.targets = c->targets.p;
+  a.sid = c->sid.p;
+  a.sys = sys_dev(c);
   a.n_targets = c->host_misc[40];
   a.n_nodes = c->n_nodes;
   a.n_parts = c->n;
@@ -287,14 +412,8 @@ int walk(Ctx* c, const bh_force_params* fp, uint32_t group_size, uint32_t target
   a.n_pp = d_npp;
   a.n_pc = d_npc;
   a.totals = c->totals.p;
-  a.theta2 = fp->theta * fp->theta;
-  a.eps2 = fp->eps2;
-  a.G = fp->G;
-  a.kick_dt = kick_dt;
-  a.drift_dt = drift_dt;
-  a.use_quad = fp->use_quadrupole != 0;
   HIP_OK(c, hipEventRecord(c->ev[4], s));
-  HIP_OK(c, launch_walk(a, c->precision == BH_FP64, s));
+  HIP_OK(c, launch_walk(a, c->precision == BH_FP64, c->any_unsoftened, s));
   HIP_OK(c, hipEventRecord(c->ev[5], s));
   c->have_stats = true;
   return BH_OK;
@@ -306,18 +425,17 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
-                uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
-                bh_step_timing* timing) {
-  TRY(check_args(c, n, fp, max_leaf, group_size));
+// One step of a batch: systems sys[0..S) own consecutive ranges of d_in.
+int step_batch(Ctx* c, const bh_particle* d_in, bh_particle* d_out, const HostSys* sys, uint32_t S, bh_step_timing* timing) {
   if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
-  if (n == 0) return BH_OK;
+  uint64_t n = 0;
   HIP_OK(c, hipSetDevice(c->device));
+  TRY(prepare_systems(c, sys, S, &n));
+  if (n == 0) return BH_OK;
   c->have_stats = false;
   HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
-  TRY(build(c, d_in, d_out, static_cast<uint32_t>(n), max_leaf));
-  c->group_size = group_size;
-  TRY(walk(c, fp, group_size, target_size, d_out, kick_dt, drift_dt, nullptr, nullptr, nullptr));
+  TRY(build(c, d_in, d_out, static_cast<uint32_t>(n)));
+  TRY(walk(c, d_out, nullptr, nullptr, nullptr));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   if (timing) {
     timing->tree_build_ms = elapsed(c->ev[0], c->ev[2]);
@@ -325,6 +443,26 @@ int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n,
     timing->exchange_ms = 0.0;
   }
   return BH_OK;
+}
+
+int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
+                uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
+                bh_step_timing* timing) {
+  TRY(check_args(c, n, fp, max_leaf, group_size));
+  if (n == 0) {
+    if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
+    return BH_OK;
+  }
+  const HostSys one{n, *fp, max_leaf, group_size, target_size, kick_dt, drift_dt};
+  return step_batch(c, d_in, d_out, &one, 1, timing);
+}
+
+std::vector<HostSys> host_systems(const bh_system* systems, uint32_t S) {
+  std::vector<HostSys> v(S);
+  for (uint32_t k = 0; k < S; ++k)
+    v[k] = HostSys{systems[k].n, systems[k].fp, systems[k].max_leaf, systems[k].group_size, systems[k].target_size,
+                   systems[k].kick_dt, systems[k].drift_dt};
+  return v;
 }
 
 Ctx* as(bh_ctx* c) { return reinterpret_cast<Ctx*>(c); }
@@ -459,6 +597,9 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->didx);
   release(c->dpart);
   release(c->dacc);
+  release(c->tables);
+  release(c->sid);
+  if (c->host_tables) (void)hipHostFree(c->host_tables);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->host_misc) (void)hipHostFree(c->host_misc);
@@ -511,6 +652,42 @@ int bh_step(bh_ctx* h, bh_particle* particles, uint64_t n, const bh_force_params
   HIP_OK(c, hipMemcpyAsync(particles, c->pb.p, n * sizeof(bh_particle), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   c->state_n = 0;  // pb no longer holds a resident state
+  return BH_OK;
+}
+
+int bh_step_batch(bh_ctx* h, const bh_particle* d_in, bh_particle* d_out, const bh_system* systems, uint32_t n_systems,
+                  bh_step_timing* timing) {
+  Ctx* c = as(h);
+  if (!c || !systems || !d_in || !d_out) return c ? (set_err(c, "null argument"), BH_ERR_INVALID) : BH_ERR_INVALID;
+  const std::vector<HostSys> v = host_systems(systems, n_systems);
+  uint64_t n = 0;
+  for (const auto& q : v) n += q.n;
+  if (d_in + n > d_out && d_out + n > d_in) {
+    set_err(c, "d_in and d_out overlap");
+    return BH_ERR_INVALID;
+  }
+  return step_batch(c, d_in, d_out, v.data(), n_systems, timing);
+}
+
+int bh_step_batch_host(bh_ctx* h, bh_particle* particles, const bh_system* systems, uint32_t n_systems,
+                       bh_step_timing* timing) {
+  Ctx* c = as(h);
+  if (!c || !systems || !particles) return c ? (set_err(c, "null argument"), BH_ERR_INVALID) : BH_ERR_INVALID;
+  const std::vector<HostSys> v = host_systems(systems, n_systems);
+  uint64_t n = 0;
+  for (const auto& q : v) n += q.n;
+  if (n > kMaxN) {
+    set_err(c, "batch of %llu particles exceeds the supported %llu", (unsigned long long)n, (unsigned long long)kMaxN);
+    return BH_ERR_TOO_LARGE;
+  }
+  HIP_OK(c, hipSetDevice(c->device));
+  TRY(ensure(c, c->tmp_in, n, "particles"));
+  TRY(ensure(c, c->pb, n, "particles"));
+  if (n) HIP_OK(c, hipMemcpyAsync(c->tmp_in.p, particles, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
+  TRY(step_batch(c, c->tmp_in.p, c->pb.p, v.data(), n_systems, timing));
+  if (n) HIP_OK(c, hipMemcpyAsync(particles, c->pb.p, n * sizeof(bh_particle), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  c->state_n = 0;
   return BH_OK;
 }
 
@@ -577,9 +754,12 @@ int bh_build_tree(bh_ctx* h, const bh_particle* host, uint64_t n, uint32_t max_l
   TRY(ensure(c, c->pb, n, "particles"));
   HIP_OK(c, hipMemcpyAsync(c->tmp_in.p, host, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
   c->have_stats = false;
+  const bh_force_params dflt{0.5, 0.0, 1.0, 1, 0};
+  const HostSys one{n, dflt, max_leaf, group_size, group_size, 0.0, 0.0};
+  uint64_t total = 0;
+  TRY(prepare_systems(c, &one, 1, &total));
   HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
-  TRY(build(c, c->tmp_in.p, c->pb.p, static_cast<uint32_t>(n), max_leaf));
-  c->group_size = group_size;
+  TRY(build(c, c->tmp_in.p, c->pb.p, static_cast<uint32_t>(n)));
   c->state_n = 0;
   HIP_OK(c, hipStreamSynchronize(c->stream));
   return BH_OK;
@@ -631,11 +811,15 @@ int bh_tree_export(bh_ctx* h, int layout, uint64_t* keys, uint32_t* perm, bh_par
     set_err(c, "unknown layout %d", layout);
     return BH_ERR_INVALID;
   }
+  if (c->n_sys != 1) {
+    set_err(c, "tree export covers single-system trees");
+    return BH_ERR_INVALID;
+  }
   HIP_OK(c, hipSetDevice(c->device));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   const uint32_t n = c->n, m = c->n_nodes;
-  if (keys) HIP_OK(c, hipMemcpy(keys, c->keys_a.p, uint64_t(n) * 8, hipMemcpyDeviceToHost));
-  if (perm) HIP_OK(c, hipMemcpy(perm, c->vals_a.p, uint64_t(n) * 4, hipMemcpyDeviceToHost));
+  if (keys) HIP_OK(c, hipMemcpy(keys, c->keys_sorted, uint64_t(n) * 8, hipMemcpyDeviceToHost));
+  if (perm) HIP_OK(c, hipMemcpy(perm, c->perm_sorted, uint64_t(n) * 4, hipMemcpyDeviceToHost));
   if (sorted) HIP_OK(c, hipMemcpy(sorted, c->sorted, uint64_t(n) * sizeof(bh_particle), hipMemcpyDeviceToHost));
   std::vector<uint32_t> begin(m), count(m), first_child(m), parent(m);
   std::vector<uint8_t> depth(m), leaf(m), nchild(m);
@@ -701,13 +885,29 @@ int bh_accel(bh_ctx* h, const bh_force_params* fp, uint32_t target_size, double*
     return BH_ERR_NO_TREE;
   }
   TRY(check_args(c, c->n, fp, c->max_leaf, c->group_size));
+  if (c->n_sys != 1) {
+    set_err(c, "bh_accel covers single-system trees");
+    return BH_ERR_INVALID;
+  }
   HIP_OK(c, hipSetDevice(c->device));
   const uint32_t n = c->n;
   DevBuf<double>& da = c->dacc;
   TRY(ensure(c, da, 3ull * n, "accelerations"));
   DevBuf<uint32_t>& di = c->didx;
   TRY(ensure(c, di, 2ull * n, "interaction counts"));
-  TRY(walk(c, fp, c->group_size, target_size, nullptr, 0.0, 0.0, da.p, di.p, di.p + n));
+  // the walk's parameters for the built tree: rewrite its system entry
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  SysDev* sd = reinterpret_cast<SysDev*>(c->host_tables);
+  sd->theta2 = fp->theta * fp->theta;
+  sd->eps2 = fp->eps2;
+  sd->G = fp->G;
+  sd->kick_dt = 0.0;
+  sd->drift_dt = 0.0;
+  sd->tsm = std::min(target_size, c->group_size);
+  sd->use_quad = fp->use_quadrupole != 0;
+  c->any_unsoftened = static_cast<float>(fp->eps2) == 0.0f;
+  HIP_OK(c, hipMemcpyAsync(c->tables.p, c->host_tables, sizeof(SysDev), hipMemcpyHostToDevice, c->stream));
+  TRY(walk(c, nullptr, da.p, di.p, di.p + n));
   if (acc) HIP_OK(c, hipMemcpyAsync(acc, da.p, 24ull * n, hipMemcpyDeviceToHost, c->stream));
   if (n_pp) HIP_OK(c, hipMemcpyAsync(n_pp, di.p, 4ull * n, hipMemcpyDeviceToHost, c->stream));
   if (n_pc) HIP_OK(c, hipMemcpyAsync(n_pc, di.p + n, 4ull * n, hipMemcpyDeviceToHost, c->stream));

@@ -22,6 +22,26 @@ constexpr uint32_t kLbAgg = 1u << 30;
 constexpr uint32_t kLbPre = 2u << 30;
 constexpr uint32_t kLbVal = (1u << 30) - 1;
 
+// One system of a batch: its particles are [off, off + n) of the batch
+// arrays, both in input and in sorted order (systems stay contiguous).
+struct SysDev {
+  double theta2;
+  double eps2;
+  double G;
+  double kick_dt;
+  double drift_dt;
+  uint32_t off;
+  uint32_t n;
+  uint32_t max_leaf;
+  uint32_t tsm;      // min(target_size, group_size)
+  int32_t use_quad;
+  int32_t pad;
+};
+static_assert(sizeof(SysDev) == 64, "SysDev layout");
+
+constexpr uint32_t kChunk = 65536;   // particles per bounds block
+constexpr uint32_t kMaxSystems = 65535;
+
 // Device view of one tree (level-order BFS numbering).
 struct TreeDev {
   uint32_t* begin;       // [M] first particle (Morton position)
@@ -56,6 +76,8 @@ struct WalkArgs {
   const uint32_t* count;
   const double4* posm;     // x, y, z, m in Morton order
   const uint32_t* targets; // target nodes, Morton order
+  const uint16_t* sid;     // system of each particle
+  const SysDev* sys;       // system table; the root of system s is node s
   uint32_t n_targets;
   uint32_t n_nodes;        // index clamps: a node or particle index never
   uint32_t n_parts;        // leaves its array, whatever the tree holds
@@ -64,40 +86,43 @@ struct WalkArgs {
   uint32_t* n_pp;          // [n] (nullable)
   uint32_t* n_pc;          // [n] (nullable)
   unsigned long long* totals;  // [2]
-  double theta2;
-  double eps2;
-  double G;
-  double kick_dt;
-  double drift_dt;
-  int use_quad;
 };
 
 // ---- bh_sort.hip
+// chunks[c] = {system, begin, end, 0}: blocks of <= kChunk particles of one
+// system; chunk_first[s] = first chunk of system s (chunk_first[S] = total).
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
-hipError_t launch_bounds(const bh_particle* p, uint32_t n, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s);
-hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const double* lohi, uint64_t* keys, uint32_t* hist,
-                            hipStream_t s);
+hipError_t launch_fill_sid(const uint4* chunks, uint32_t n_chunks, uint16_t* sid, hipStream_t s);
+hipError_t launch_bounds(const bh_particle* p, const uint4* chunks, uint32_t n_chunks, const uint32_t* chunk_first,
+                         uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s);
+hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
+                            uint32_t* hist, hipStream_t s);
+// 8 passes over the 63-bit keys, then sid_passes (0-2) over the system id
+// taken from sid[index]: a stable sort of each system's (key, index) pairs
+// in place. hist holds 10 x 256 digit counts (passes 8-9: the systems).
 hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b, uint32_t n,
-                             uint32_t* hist, uint32_t* digit_base, uint32_t* lookback, uint32_t* tile_counters,
+                             int sid_passes, const uint16_t* sid, uint32_t* hist, uint32_t* digit_base,
+                             uint32_t* lookback, uint32_t* tile_counters, uint64_t** keys_out, uint32_t** vals_out,
                              hipStream_t s);
 hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
                          hipStream_t s);
-uint64_t radix_lookback_words(uint32_t n);
+uint64_t radix_lookback_words(uint32_t n);  // 10 passes of look-back words + 10 tile counters
 
 // ---- bh_tree.hip
-hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, uint32_t max_leaf, uint8_t* L, uint8_t* H,
-                             uint32_t* tile_counts, hipStream_t s);
+hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, const uint16_t* sid, const SysDev* sys,
+                             uint32_t max_leaf_max, uint8_t* L, uint8_t* H, uint32_t* tile_counts, hipStream_t s);
 hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_t* tile_prefix, uint32_t* totals,
                             hipStream_t s);
 hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix,
                             const uint32_t* totals, TreeDev t, hipStream_t s);
-hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const double* lohi,
-                              const uint32_t* totals, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s);
-hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, uint32_t tsm, uint32_t* mark, uint32_t* tile_counts,
-                          uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets, hipStream_t s);
+hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid,
+                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s);
+hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, uint32_t* mark,
+                          uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets,
+                          hipStream_t s);
 
 // ---- bh_walk.hip
-hipError_t launch_walk(const WalkArgs& a, int fp64, hipStream_t s);
+hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s);
 hipError_t launch_direct(const double4* src, uint32_t n, const uint32_t* tidx, uint32_t m, double eps2, double G,
                          double* partial, uint32_t splits, double* acc, hipStream_t s);
 uint32_t direct_splits(uint32_t n, uint32_t m);

@@ -76,14 +76,22 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// Per-block min/max of the positions; flags any non-finite position,
-// velocity or mass.
-__global__ __launch_bounds__(kBlock) void k_bounds_partial(const bh_particle* __restrict__ p, uint32_t n,
+// System id of every particle, one block per chunk.
+__global__ __launch_bounds__(kBlock) void k_fill_sid(const uint4* __restrict__ chunks, uint16_t* __restrict__ sid) {
+  const uint4 c = chunks[blockIdx.x];
+  for (uint32_t i = c.y + threadIdx.x; i < c.z; i += kBlock) sid[i] = static_cast<uint16_t>(c.x);
+}
+
+// Min/max of the positions of one chunk (flat_build_ttg.h:93-113; min/max
+// is exact, so the chunking does not matter); flags any non-finite
+// position, velocity or mass.
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const bh_particle* __restrict__ p, const uint4* __restrict__ chunks,
                                                            double* __restrict__ partial, uint32_t* __restrict__ nonfinite) {
   const double inf = __builtin_huge_val();
   double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
   bool bad = false;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  const uint4 c = chunks[blockIdx.x];
+  for (uint32_t i = c.y + threadIdx.x; i < c.z; i += kBlock) {
     const bh_particle& q = p[i];
     const double x = q.pos[0], y = q.pos[1], z = q.pos[2];
     bad |= !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(q.mass) && isfinite(q.vel[0]) && isfinite(q.vel[1]) &&
@@ -114,12 +122,14 @@ __global__ __launch_bounds__(kBlock) void k_bounds_partial(const bh_particle* __
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bounds_final(const double* __restrict__ partial, uint32_t blocks,
-                                                         double* __restrict__ lohi) {
+// Per system: reduce its chunks' partials to lohi[6 s .. 6 s + 6).
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const double* __restrict__ partial,
+                                                         const uint32_t* __restrict__ chunk_first, double* __restrict__ lohi) {
   const double inf = __builtin_huge_val();
   __shared__ double s[4][6];
   double r[6] = {inf, inf, inf, -inf, -inf, -inf};
-  for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
+  const uint32_t b0 = chunk_first[blockIdx.x], b1 = chunk_first[blockIdx.x + 1];
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
     for (int a = 0; a < 3; ++a) {
       r[a] = fmin(r[a], partial[b * 6 + a]);
       r[3 + a] = fmax(r[3 + a], partial[b * 6 + 3 + a]);
@@ -137,23 +147,23 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const double* __restric
     const int a = threadIdx.x;
     double v = s[0][a];
     for (int k = 1; k < 4; ++k) v = a < 3 ? fmin(v, s[k][a]) : fmax(v, s[k][a]);
-    lohi[a] = v;
+    lohi[6 * blockIdx.x + a] = v;
   }
 }
 
 // Morton key per particle (input order) and the 8 digit histograms of all
 // keys, for the onesweep passes' global digit offsets.
 __global__ __launch_bounds__(kBlock) void k_keys_hist(const bh_particle* __restrict__ p, uint32_t n,
-                                                      const double* __restrict__ lohi, uint64_t* __restrict__ keys,
-                                                      uint32_t* __restrict__ hist) {
+                                                      const uint16_t* __restrict__ sid, const double* __restrict__ lohi,
+                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[8 * 256];
   for (int i = threadIdx.x; i < 8 * 256; i += kBlock) s_h[i] = 0;
-  double c[3], hw;
-  domain_from_lohi(lohi, c, &hw);
-  const double scale = 2097152.0 / (2.0 * hw);
-  const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
   __syncthreads();
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double c[3], hw;
+    domain_from_lohi(lohi + 6 * sid[i], c, &hw);
+    const double scale = 2097152.0 / (2.0 * hw);
+    const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
     const bh_particle& q = p[i];
     const uint64_t key = spread_bits3(quantize(q.pos[0], lx, scale)) | (spread_bits3(quantize(q.pos[1], ly, scale)) << 1) |
                          (spread_bits3(quantize(q.pos[2], lz, scale)) << 2);
@@ -193,16 +203,20 @@ __global__ __launch_bounds__(kBlock) void k_hist_scan(const uint32_t* __restrict
   dbase[pass * 256 + threadIdx.x] = block_excl_scan(v, s_w, nullptr);
 }
 
-template <bool kFirst>
+// kMode 0: first key pass (payload = input index); 1: key pass; 2: system
+// pass (digit of sid[payload], keys carried along).
+template <int kMode>
 __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                      uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
-                                                     uint32_t shift, const uint32_t* __restrict__ dbase,
-                                                     uint32_t* __restrict__ lookback, uint32_t* __restrict__ tile_counter) {
+                                                     uint32_t shift, const uint16_t* __restrict__ sid,
+                                                     const uint32_t* __restrict__ dbase, uint32_t* __restrict__ lookback,
+                                                     uint32_t* __restrict__ tile_counter) {
   __shared__ uint32_t s_hist[4][256];   // per-wave digit counters, then per-wave exclusive bases
   __shared__ uint32_t s_lexcl[256];     // tile-local exclusive digit offsets
   __shared__ uint32_t s_adj[256];       // global position - local position, per digit
   __shared__ uint64_t s_k[kSortTile];
   __shared__ uint32_t s_v[kSortTile];
+  __shared__ uint8_t s_d[kSortTile];
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_tile;
 
@@ -216,12 +230,17 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
   uint64_t k[kItems];
   uint32_t v[kItems];
   uint32_t r[kItems];
+  uint32_t dg[kItems];
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
     const bool ok = idx < n;
     k[j] = ok ? kin[idx] : ~0ull;
-    v[j] = kFirst ? idx : (ok ? vin[idx] : 0u);
+    v[j] = kMode == 0 ? idx : (ok ? vin[idx] : 0u);
+    if (kMode == 2)
+      dg[j] = ok ? (static_cast<uint32_t>(sid[v[j]]) >> shift) & 255u : 255u;
+    else
+      dg[j] = static_cast<uint32_t>(k[j] >> shift) & 255u;
   }
   const uint64_t lt = lanemask_lt();
   // Stable in-tile ranking: items in (wave, item, lane) order follow the
@@ -230,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
   for (int j = 0; j < kItems; ++j) {
     const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
     const bool ok = idx < n;
-    const uint32_t d = static_cast<uint32_t>(k[j] >> shift) & 255u;
+    const uint32_t d = dg[j];
     uint64_t peers = __ballot(ok);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -280,19 +299,18 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
   for (int j = 0; j < kItems; ++j) {
     const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
     if (idx < n) {
-      const uint32_t d = static_cast<uint32_t>(k[j] >> shift) & 255u;
+      const uint32_t d = dg[j];
       const uint32_t pos = s_lexcl[d] + s_hist[w][d] + r[j];
       s_k[pos] = k[j];
       s_v[pos] = v[j];
+      s_d[pos] = static_cast<uint8_t>(d);
     }
   }
   __syncthreads();
   const uint32_t tn = n - base < kSortTile ? n - base : kSortTile;
   for (uint32_t e = tid; e < tn; e += kBlock) {
-    const uint64_t key = s_k[e];
-    const uint32_t d = static_cast<uint32_t>(key >> shift) & 255u;
-    const uint32_t g = s_adj[d] + e;
-    kout[g] = key;
+    const uint32_t g = s_adj[s_d[e]] + e;
+    kout[g] = s_k[e];
     vout[g] = s_v[e];
   }
 }
@@ -333,42 +351,53 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_bounds(const bh_particle* p, uint32_t n, double* partial, uint32_t* nonfinite, double* lohi,
-                         hipStream_t s) {
-  const uint32_t g = grid_for(n, 1024);
-  hipLaunchKernelGGL(k_bounds_partial, dim3(g), dim3(kBlock), 0, s, p, n, partial, nonfinite);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBlock), 0, s, partial, g, lohi);
+hipError_t launch_fill_sid(const uint4* chunks, uint32_t n_chunks, uint16_t* sid, hipStream_t s) {
+  if (n_chunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_sid, dim3(n_chunks), dim3(kBlock), 0, s, chunks, sid);
   return hipGetLastError();
 }
 
-hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const double* lohi, uint64_t* keys, uint32_t* hist,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_keys_hist, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, p, n, lohi, keys, hist);
+hipError_t launch_bounds(const bh_particle* p, const uint4* chunks, uint32_t n_chunks, const uint32_t* chunk_first,
+                         uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds_partial, dim3(n_chunks), dim3(kBlock), 0, s, p, chunks, partial, nonfinite);
+  hipLaunchKernelGGL(k_bounds_final, dim3(n_sys), dim3(kBlock), 0, s, partial, chunk_first, lohi);
+  return hipGetLastError();
+}
+
+hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
+                            uint32_t* hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys_hist, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, p, n, sid, lohi, keys, hist);
   return hipGetLastError();
 }
 
 uint64_t radix_lookback_words(uint32_t n) {
   const uint64_t tiles = (uint64_t(n) + kSortTile - 1) / kSortTile;
-  return 8 * tiles * 256 + 8;  // 8 passes of tiles x 256 words, then 8 tile counters
+  return 10 * tiles * 256 + 10;
 }
 
 hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_b, uint32_t* vals_b, uint32_t n,
-                             uint32_t* hist, uint32_t* digit_base, uint32_t* lookback, uint32_t* tile_counters,
+                             int sid_passes, const uint16_t* sid, uint32_t* hist, uint32_t* digit_base,
+                             uint32_t* lookback, uint32_t* tile_counters, uint64_t** keys_out, uint32_t** vals_out,
                              hipStream_t s) {
   const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-  hipLaunchKernelGGL(k_hist_scan, dim3(8), dim3(kBlock), 0, s, hist, digit_base);
+  const int passes = 8 + sid_passes;
+  hipLaunchKernelGGL(k_hist_scan, dim3(passes), dim3(kBlock), 0, s, hist, digit_base);
   uint64_t* kin = keys_a;
   uint32_t* vin = vals_a;
   uint64_t* ko = keys_b;
   uint32_t* vo = vals_b;
-  for (int pass = 0; pass < 8; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     uint32_t* lb = lookback + uint64_t(pass) * tiles * 256;
+    const uint32_t* db = digit_base + pass * 256;
     if (pass == 0)
-      hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 8u * pass,
-                         digit_base + pass * 256, lb, tile_counters + pass);
+      hipLaunchKernelGGL(k_onesweep<0>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 0u, sid, db, lb,
+                         tile_counters + pass);
+    else if (pass < 8)
+      hipLaunchKernelGGL(k_onesweep<1>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 8u * pass, sid, db, lb,
+                         tile_counters + pass);
     else
-      hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 8u * pass,
-                         digit_base + pass * 256, lb, tile_counters + pass);
+      hipLaunchKernelGGL(k_onesweep<2>, dim3(tiles), dim3(kBlock), 0, s, kin, vin, ko, vo, n, 8u * (pass - 8), sid, db,
+                         lb, tile_counters + pass);
     uint64_t* tk = kin;
     kin = ko;
     ko = tk;
@@ -376,7 +405,8 @@ hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_
     vin = vo;
     vo = tv;
   }
-  // 8 passes: the sorted keys and indices end in (keys_a, vals_a)
+  *keys_out = kin;
+  *vals_out = vin;
   return hipGetLastError();
 }
 

@@ -26,6 +26,11 @@
 // cs(n) = number of level-(d+1) begins before begin(n), n's children are
 // [cs(n), cs(next node of level d)).
 //
+// A batch of independent systems is built in the same launches: each
+// system's particles are contiguous, its first particle begins its root
+// (level-0 node s for system s), and the windows stop at its boundaries, so
+// every system gets exactly the tree it would get alone.
+//
 // Compiled with -ffp-contract=off: centres and multipoles round exactly as
 // the reference's FP64 code does.
 #include "bh_internal.h"
@@ -62,36 +67,47 @@ __device__ inline void domain_from_lohi(const double* lohi, double c[3], double*
   *hw = h;
 }
 
-__global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restrict__ keys, uint32_t n, uint32_t ml,
-                                                       uint8_t* __restrict__ L, uint8_t* __restrict__ H,
+// L, H per particle and per-tile counts of level begins and leaf begins.
+// With several systems the windows stop at system boundaries: a system's
+// first particle begins its root (L = 0) and P, W only look at particles of
+// the same system, with that system's max_leaf.
+__global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restrict__ keys, uint32_t n,
+                                                       const uint16_t* __restrict__ sid, const SysDev* __restrict__ sys,
+                                                       uint32_t mlx, uint8_t* __restrict__ L, uint8_t* __restrict__ H,
                                                        uint32_t* __restrict__ tile_counts) {
   __shared__ uint64_t s_k[kTreeTile + 2 * kMaxLeafCap + 1];
   __shared__ int8_t s_p[kTreeTile + kMaxLeafCap];
   __shared__ uint32_t s_cnt[kTreeCols];
   const uint32_t tile = blockIdx.x, base = tile * kTreeTile;
-  const int64_t g0 = static_cast<int64_t>(base) - ml - 1;  // global index of s_k[0]
-  const uint32_t wk = kTreeTile + 2 * ml + 1;
+  const int64_t g0 = static_cast<int64_t>(base) - mlx - 1;  // global index of s_k[0]
+  const uint32_t wk = kTreeTile + 2 * mlx + 1;
   for (uint32_t e = threadIdx.x; e < wk; e += kBlock) {
     const int64_t g = g0 + e;
     s_k[e] = (g >= 0 && g < n) ? keys[g] : 0ull;
   }
   if (threadIdx.x < kTreeCols) s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  // P_u for u in [base - ml, base + tile)
-  for (uint32_t e = threadIdx.x; e < kTreeTile + ml; e += kBlock) {
-    const int64_t u = static_cast<int64_t>(base) - ml + e;
+  // P_u for u in [base - mlx, base + tile)
+  for (uint32_t e = threadIdx.x; e < kTreeTile + mlx; e += kBlock) {
+    const int64_t u = static_cast<int64_t>(base) - mlx + e;
     int8_t pv = -1;
-    if (u >= 0 && u + ml < n) pv = static_cast<int8_t>(common_digits(s_k[u - g0], s_k[u + ml - g0]));
+    if (u >= 0 && u < n) {
+      const SysDev& q = sys[sid[u]];
+      if (u + q.max_leaf < q.off + q.n) pv = static_cast<int8_t>(common_digits(s_k[u - g0], s_k[u + q.max_leaf - g0]));
+    }
     s_p[e] = pv;
   }
   __syncthreads();
   for (uint32_t e = threadIdx.x; e < kTreeTile; e += kBlock) {
     const uint32_t i = base + e;
     if (i >= n) break;
-    const int li = (i == 0) ? 0 : common_digits(s_k[i - 1 - g0], s_k[i - g0]) + 1;
+    const SysDev& q = sys[sid[i]];
+    const uint32_t ml = q.max_leaf;
+    const int li = (i == q.off) ? 0 : common_digits(s_k[i - 1 - g0], s_k[i - g0]) + 1;
+    const uint32_t span = (i - q.off < ml) ? i - q.off : ml;  // u = i - d, d <= span, stays in the system
     int wmax = -1;
-    for (uint32_t u = 0; u <= ml; ++u) wmax = max(wmax, static_cast<int>(s_p[e + u]));
-    const int hi = (n > ml) ? min(wmax, kDepthCap - 1) + 1 : 0;
+    for (uint32_t d = 0; d <= span; ++d) wmax = max(wmax, static_cast<int>(s_p[e + mlx - d]));
+    const int hi = (q.n > ml) ? min(wmax, kDepthCap - 1) + 1 : 0;
     L[i] = static_cast<uint8_t>(li);
     H[i] = static_cast<uint8_t>(hi);
     for (int d = li; d <= hi; ++d) atomicAdd(&s_cnt[d], 1u);
@@ -221,16 +237,16 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit(const uint8_t* __restrict_
 
 // Children, parents, leaf counts and cell geometry of every node.
 __global__ __launch_bounds__(kBlock) void k_tree_links(const uint64_t* __restrict__ keys, uint32_t n,
-                                                       const double* __restrict__ lohi, uint32_t n_nodes, uint32_t n_leaves,
-                                                       TreeDev t) {
-  double c0[3], hw0;
-  domain_from_lohi(lohi, c0, &hw0);
+                                                       const uint16_t* __restrict__ sid, const double* __restrict__ lohi,
+                                                       uint32_t n_nodes, uint32_t n_leaves, TreeDev t) {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
     const int d = t.depth[i];
     const uint32_t lend = t.level_off[d + 1];
     const uint32_t next = (i + 1 < lend) ? t.cs[i + 1] : t.level_off[d + 2];
     const uint32_t cs = t.cs[i];
-    if (i == 0) t.parent[0] = 0xffffffffu;
+    if (d == 0) t.parent[i] = 0xffffffffu;  // the root of a system
+    double c0[3], hw0;
+    domain_from_lohi(lohi + 6 * sid[t.begin[i]], c0, &hw0);
     if (t.leaf[i]) {
       t.first_child[i] = 0;
       t.nchild[i] = 0;
@@ -336,11 +352,13 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
 // Marks each walk target at its first particle: topmost nodes with
 // leaf || count <= tsm (tsm = min(target_size, group_size), which makes the
 // group-root / target nesting of grouped_ttg.h:251-288 a single predicate).
-__global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_nodes, uint32_t tsm, uint32_t* __restrict__ mark) {
+__global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_nodes, const uint16_t* __restrict__ sid,
+                                                        const SysDev* __restrict__ sys, uint32_t* __restrict__ mark) {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
+    const uint32_t tsm = sys[sid[t.begin[i]]].tsm;
     const bool pred = t.leaf[i] || t.count[i] <= tsm;
     if (!pred) continue;
-    const bool ppred = (i != 0) && (t.count[t.parent[i]] <= tsm);
+    const bool ppred = (t.depth[i] != 0) && (t.count[t.parent[i]] <= tsm);
     if (!ppred) mark[t.begin[i]] = i + 1;
   }
 }
@@ -396,10 +414,10 @@ inline uint32_t grid_for(uint64_t n, uint32_t cap) {
 
 }  // namespace
 
-hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, uint32_t max_leaf, uint8_t* L, uint8_t* H,
-                             uint32_t* tile_counts, hipStream_t s) {
+hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, const uint16_t* sid, const SysDev* sys,
+                             uint32_t max_leaf_max, uint8_t* L, uint8_t* H, uint32_t* tile_counts, hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
-  hipLaunchKernelGGL(k_tree_count, dim3(tiles), dim3(kBlock), 0, s, keys, n, max_leaf, L, H, tile_counts);
+  hipLaunchKernelGGL(k_tree_count, dim3(tiles), dim3(kBlock), 0, s, keys, n, sid, sys, max_leaf_max, L, H, tile_counts);
   return hipGetLastError();
 }
 
@@ -417,10 +435,10 @@ hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, cons
   return hipGetLastError();
 }
 
-hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const double* lohi,
-                              const uint32_t* totals, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s) {
-  (void)totals;
-  hipLaunchKernelGGL(k_tree_links, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, keys, n, lohi, n_nodes, n_leaves, t);
+hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid,
+                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s) {
+  hipLaunchKernelGGL(k_tree_links, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, keys, n, sid, lohi, n_nodes, n_leaves,
+                     t);
   // bottom-up, one launch per level; each reads its level range on the device
   for (int d = kDepthCap; d >= 0; --d) {
     hipLaunchKernelGGL(k_upward, dim3(grid_for(n_nodes, 2048)), dim3(kBlock), 0, s, posm, d, t);
@@ -428,12 +446,13 @@ hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, uint32_t tsm, uint32_t* mark, uint32_t* tile_counts,
-                          uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets, hipStream_t s) {
+hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, uint32_t* mark,
+                          uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets,
+                          hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   hipError_t e = launch_fill_u32(mark, 0u, n, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, tsm, mark);
+  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys, mark);
   hipLaunchKernelGGL(k_compact_count, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_counts);
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, 1u, tile_prefix, n_targets);
   hipLaunchKernelGGL(k_compact_write, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_prefix, targets);

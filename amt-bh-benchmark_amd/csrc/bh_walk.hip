@@ -223,6 +223,10 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
   const int lane = __lane_id();
   const uint32_t tb = uniform(a.begin[tnode]);
   const uint32_t T = uniform(a.count[tnode]);
+  const uint32_t sys_id = uniform(a.sid[tb]);
+  const SysDev& sp = a.sys[sys_id];
+  const double sys_G = sp.G, kick_dt = sp.kick_dt, drift_dt = sp.drift_dt;
+  const bool use_quad = sp.use_quad != 0;
   const double4 tcv = a.ctr[tnode];
   const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
   const uint32_t last_node = a.n_nodes - 1, last_part = a.n_parts - 1;
@@ -244,8 +248,8 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
   bhi0 = wave_max(bhi0);
   bhi1 = wave_max(bhi1);
   bhi2 = wave_max(bhi2);
-  const double theta2 = a.theta2;
-  const Real eps2 = static_cast<Real>(a.eps2);
+  const double theta2 = sp.theta2;
+  const Real eps2 = static_cast<Real>(sp.eps2);
   const uint64_t lt = lanemask_lt();
 
   for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
@@ -305,7 +309,7 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
     };
 
     if (lane == 0) {
-      L.fr[0][0] = 0u;  // node 0, one node
+      L.fr[0][0] = sys_id << 3;  // the system's root: node sys_id, one node
       L.fr_n[0] = 1;
       L.fr_cur[0] = 0;
     }
@@ -375,7 +379,7 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
       if (cm) {
         const uint32_t nc = __popcll(cm);
         const uint32_t pos = __popcll(cm & lt);
-        if (a.use_quad) {
+        if (use_quad) {
           if (cellf) {
             const uint32_t slot = pc_cnt + pos;
             if constexpr (sizeof(Real) == 4) {
@@ -470,7 +474,7 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
         ay += red[3 * src + 1];
         az += red[3 * src + 2];
       }
-      const double gx = a.G * ax, gy = a.G * ay, gz = a.G * az;
+      const double gx = sys_G * ax, gy = sys_G * ay, gz = sys_G * az;
       const uint32_t k3 = tb + c0 + lane;
       if (a.acc) {
         a.acc[3 * uint64_t(k3) + 0] = gx;
@@ -482,16 +486,16 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
       if (a.parts) {
         // kick_one / drift_one (integrate.h:15-16): v += a*dt; x += v*dt
         bh_particle& pt = a.parts[k3];
-        const double vx = pt.vel[0] + gx * a.kick_dt;
-        const double vy = pt.vel[1] + gy * a.kick_dt;
-        const double vz = pt.vel[2] + gz * a.kick_dt;
+        const double vx = pt.vel[0] + gx * kick_dt;
+        const double vy = pt.vel[1] + gy * kick_dt;
+        const double vz = pt.vel[2] + gz * kick_dt;
         pt.vel[0] = vx;
         pt.vel[1] = vy;
         pt.vel[2] = vz;
-        if (a.drift_dt != 0.0) {
-          pt.pos[0] = pt.pos[0] + vx * a.drift_dt;
-          pt.pos[1] = pt.pos[1] + vy * a.drift_dt;
-          pt.pos[2] = pt.pos[2] + vz * a.drift_dt;
+        if (drift_dt != 0.0) {
+          pt.pos[0] = pt.pos[0] + vx * drift_dt;
+          pt.pos[1] = pt.pos[1] + vy * drift_dt;
+          pt.pos[2] = pt.pos[2] + vz * drift_dt;
         }
       }
     }
@@ -610,11 +614,13 @@ __global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ s
   if (threadIdx.x == 0) partial[uint64_t(split) * gridDim.x + blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
 }
 
+// Masking r = 0 is needed only without softening (grouped_ttg.h:119-123); it
+// never triggers when eps2 > 0, so one masked launch serves mixed batches.
 template <typename Real, int kWaves>
-hipError_t launch_walk_t(const WalkArgs& a, hipStream_t s) {
+hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) {
   uint32_t blocks = (a.n_targets + kWaves - 1) / kWaves;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
-  if (a.eps2 > 0.0)
+  if (!any_unsoftened)
     hipLaunchKernelGGL((k_walk<Real, kWaves, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
   else
     hipLaunchKernelGGL((k_walk<Real, kWaves, true>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
@@ -623,9 +629,9 @@ hipError_t launch_walk_t(const WalkArgs& a, hipStream_t s) {
 
 }  // namespace
 
-hipError_t launch_walk(const WalkArgs& a, int fp64, hipStream_t s) {
+hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s) {
   if (a.n_targets == 0) return hipSuccess;
-  return fp64 ? launch_walk_t<double, 2>(a, s) : launch_walk_t<float, 2>(a, s);
+  return fp64 ? launch_walk_t<double, 2>(a, any_unsoftened, s) : launch_walk_t<float, 2>(a, any_unsoftened, s);
 }
 
 hipError_t launch_potential(const double4* src, uint32_t n, double* partial, uint32_t splits, hipStream_t s) {

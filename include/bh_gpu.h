@@ -115,6 +115,31 @@ int bh_step_device(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, uin
                    uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
                    bh_step_timing* timing);
 
+/* One independent system of a batch: n particles (consecutive in the batch
+ * arrays, in table order) and its own step_particles arguments. */
+typedef struct bh_system {
+  uint64_t n;
+  bh_force_params fp;
+  uint32_t max_leaf;
+  uint32_t group_size;
+  uint32_t target_size;
+  uint32_t pad;
+  double kick_dt;
+  double drift_dt;
+} bh_system;
+
+/* step_particles for many independent systems in the same launches (no
+ * reference counterpart: the batched path of one GPU). d_in holds the
+ * systems' particles back to back in table order; d_out receives each
+ * system's updated particles in its own Morton order, in the same ranges.
+ * Every system's result is bit-identical to a bh_step_device call on it
+ * alone. At most 65,535 systems. */
+int bh_step_batch(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, const bh_system* systems,
+                  uint32_t n_systems, bh_step_timing* timing);
+/* The same on host memory, in place. */
+int bh_step_batch_host(bh_ctx* ctx, bh_particle* particles, const bh_system* systems, uint32_t n_systems,
+                       bh_step_timing* timing);
+
 /* Particles resident in device memory across steps (the caller's
  * GroupedWorkspace reuse, main.cc:203-242, without host copies). */
 int bh_state_upload(bh_ctx* ctx, const bh_particle* host, uint64_t n);

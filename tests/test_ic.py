@@ -61,3 +61,49 @@ def test_uniform_cube():
     assert p["pos"].min() >= 0.0 and p["pos"].max() < 1.0
     assert np.allclose(p["pos"].mean(0), 0.5, atol=5e-3)
     assert np.all(p["vel"] == 0) and abs(p["mass"].sum() - 1) < 1e-12
+
+
+def test_hernquist_df_reproduces_density():
+    """The distribution function integrates to the model's density
+    rho(r) = 1 / (2 pi r (1 + r)^3) (pins the DF formula)."""
+    from scipy.integrate import quad
+
+    for r in (0.05, 0.3, 1.0, 4.0, 15.0):
+        psi = 1.0 / (1.0 + r)
+        vesc = np.sqrt(2.0 * psi)
+        rho = quad(lambda v: 4 * np.pi * v * v * ic.hernquist_df(psi - 0.5 * v * v), 0, vesc, epsabs=0, epsrel=1e-10)[0]
+        want = 1.0 / (2 * np.pi * r * (1 + r) ** 3)
+        assert abs(rho / want - 1) < 1e-6, (r, rho, want)
+
+
+def test_hernquist_distribution():
+    from scipy.integrate import quad
+
+    n = 200000
+    p = ic.hernquist(n, 5)
+    r = np.linalg.norm(p["pos"], axis=1)
+    assert r.max() < 20.0 and abs(p["mass"].sum() - 1.0) < 1e-12
+    norm = 400.0 / 441.0
+    for x in (0.2, 1.0, 3.0, 10.0):
+        assert abs((r < x).mean() - (x * x / (1 + x) ** 2) / norm) < 5e-3
+    v = np.linalg.norm(p["vel"], axis=1)
+    psi = 1.0 / (1.0 + r)
+    assert np.all(v * v < 2 * psi)
+    # mean v^2 in radial shells against the DF's second moment
+    for a, b in ((0.1, 0.2), (0.9, 1.1), (4.0, 6.0)):
+        sel = (r >= a) & (r < b)
+        rm = np.median(r[sel])
+        ps = 1.0 / (1.0 + rm)
+        f = lambda v, k: v ** k * ic.hernquist_df(ps - 0.5 * v * v)
+        vmax = np.sqrt(2 * ps)
+        want = quad(f, 0, vmax, args=(4,))[0] / quad(f, 0, vmax, args=(2,))[0]
+        got = (v[sel] ** 2).mean()
+        assert abs(got / want - 1) < 0.03, (a, b, got, want)
+    assert np.all(np.abs(p["vel"].mean(0)) < 0.01)
+
+
+def test_hernquist_deterministic_prefix_stable():
+    a = ic.hernquist(500, 9)
+    b = ic.hernquist(800, 9)
+    assert np.array_equal(a["pos"], b["pos"][:500]) and np.array_equal(a["vel"], b["vel"][:500])
+    assert np.array_equal(a, ic.hernquist(500, 9))
