@@ -23,12 +23,20 @@ angles (slots 1024, 1025). Masses 1/N.
 Hernquist (G = M = a = 1, mixed-N ensemble): radius by inverse CDF of
 M(<r) = r^2 / (1 + r)^2, r = sqrt(u) / (1 - sqrt(u)), redrawn until r < 20 a
 (slots 0..63); isotropic position angles (slots 64, 65); speed from the
-isotropic ergodic distribution function (Hernquist 1990, ApJ 356, 359,
-eq. 17) by rejection in s = v / v_esc: s, y uniform, accept when
-y * B(r) < s^2 f(psi (1 - s^2)) with psi = 1 / (1 + r) and B(r) 1.25 times
-the largest value of s^2 f(psi (1 - s^2)) over a 257-point grid of s (slot
-pairs 128+2k, 129+2k); isotropic velocity angles (slots 1024, 1025).
-Masses 1/N.
+isotropic ergodic distribution function f(E) = b(q) / (8 sqrt(2) pi^3
+(1 - q^2)^(5/2)), q = sqrt(E), b(q) = 3 asin q + q sqrt(1 - q^2) (1 - 2q^2)
+(8q^4 - 8q^2 - 3) (Hernquist 1990, ApJ 356, 359, eq. 17), as s = v / v_esc
+with density ~ g(s) = s^2 f(psi (1 - s^2)), psi = 1 / (1 + r), drawn by
+rejection with slot pairs (128+2k, 129+2k) = (x, y):
+  r < 1:  proposal h(s) ~ s^2 (r + s^2)^(-5/2) (g's shape as E -> 1), by
+          inversion s = t sqrt(r / (1 - t^2)), t = x^(1/3) / sqrt(1 + r);
+          since psi (r + s^2) = 1 - E and b <= b(1) = 3 pi / 2, g / h is
+          b(q) / (3 pi / 2) up to a constant: accept when y < b(q) / (3 pi / 2);
+  r >= 1: s = x, accept when y B(r) < g(s), B(r) = 1.25 max g over 257
+          equally spaced s at the table radius r_j = 20^(j / 4096) <= r,
+          j = floor(4096 ln r / ln 20) (f grows with E, so g at r_j
+          dominates g at r).
+Isotropic velocity angles (slots 1024, 1025). Masses 1/N.
 """
 import numpy as np
 
@@ -140,14 +148,32 @@ def random_box(n, seed, spread, center=(0.0, 0.0, 0.0), vel_spread=0.0):
     return p
 
 
+def _hernquist_b(q):
+    q2 = q * q
+    return 3.0 * np.arcsin(q) + q * np.sqrt(1.0 - q2) * (1.0 - 2.0 * q2) * (8.0 * q2 * q2 - 8.0 * q2 - 3.0)
+
+
 def hernquist_df(E):
     """Isotropic distribution function of the Hernquist model, G = M = a = 1
     (Hernquist 1990, eq. 17), for relative binding energies 0 <= E < 1."""
     E = np.asarray(E, dtype=np.float64)
     q = np.sqrt(np.clip(E, 0.0, 1.0 - 1e-16))
-    q2 = q * q
-    core = 3.0 * np.arcsin(q) + q * np.sqrt(1.0 - q2) * (1.0 - 2.0 * q2) * (8.0 * q2 * q2 - 8.0 * q2 - 3.0)
-    return np.maximum(core, 0.0) / ((1.0 - q2) ** 2.5 * 8.0 * np.sqrt(2.0) * np.pi ** 3)
+    return np.maximum(_hernquist_b(q), 0.0) / ((1.0 - q * q) ** 2.5 * 8.0 * np.sqrt(2.0) * np.pi ** 3)
+
+
+_BOUND = []
+_NODES = 4096
+
+
+def hernquist_bound(r):
+    """Rejection bound B(r) of the speed draw for radii 1 <= r <= 20."""
+    if not _BOUND:
+        rj = 20.0 ** (np.arange(_NODES + 1) / _NODES)
+        s = np.linspace(0.0, 1.0, 257)
+        psi = 1.0 / (1.0 + rj[:, None])
+        _BOUND.append(1.25 * (s * s * hernquist_df(psi * (1.0 - s * s))).max(axis=1))
+    j = np.floor(_NODES * np.log(np.asarray(r, dtype=np.float64)) / np.log(20.0)).astype(np.int64)
+    return _BOUND[0][np.clip(j, 0, _NODES)]
 
 
 def hernquist(n, seed, rmax=20.0):
@@ -170,11 +196,8 @@ def hernquist(n, seed, rmax=20.0):
         raise RuntimeError("hernquist: radius rejection did not converge")
     pos = _isotropic(st, 64) * r[:, None]
     psi = 1.0 / (1.0 + r)
-    grid = np.linspace(0.0, 1.0, 257)
-    bound = np.empty(n)
-    for lo in range(0, n, 1 << 16):
-        ps = psi[lo:lo + (1 << 16), None]
-        bound[lo:lo + (1 << 16)] = 1.25 * (grid * grid * hernquist_df(ps * (1.0 - grid * grid))).max(axis=1)
+    inner = r < 1.0
+    bound = np.where(inner, 0.0, hernquist_bound(np.maximum(r, 1.0)))
     s_v = np.full(n, np.nan)
     todo = np.arange(n)
     for k in range(400):
@@ -182,8 +205,15 @@ def hernquist(n, seed, rmax=20.0):
             break
         x = uniform(st[todo], 128 + 2 * k)
         y = uniform(st[todo], 129 + 2 * k)
-        ok = y * bound[todo] < x * x * hernquist_df(psi[todo] * (1.0 - x * x))
-        s_v[todo[ok]] = x[ok]
+        rt, pt, it = r[todo], psi[todo], inner[todo]
+        t = np.cbrt(x) / np.sqrt(1.0 + rt)
+        s_in = t * np.sqrt(rt / (1.0 - t * t))
+        sv = np.where(it, s_in, x)
+        e = pt * (1.0 - sv * sv)
+        ok_in = y < _hernquist_b(np.sqrt(np.clip(e, 0.0, 1.0))) / (1.5 * np.pi)
+        ok_out = y * bound[todo] < sv * sv * hernquist_df(e)
+        ok = np.where(it, ok_in, ok_out)
+        s_v[todo[ok]] = sv[ok]
         todo = todo[~ok]
     if todo.size:
         raise RuntimeError("hernquist: velocity rejection did not converge")

@@ -43,6 +43,7 @@ SIGNATURES = {
     "bh_step_device": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_step_batch": ([_P, _P, _P, _P, _U32, _P], _I),
     "bh_step_batch_host": ([_P, _P, _P, _U32, _P], _I),
+    "bh_state_step_batch": ([_P, _P, _U32, _P], _I),
     "bh_state_upload": ([_P, _P, _U64], _I),
     "bh_state_step": ([_P, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_state_download": ([_P, _P, _U64], _I),
@@ -57,6 +58,15 @@ SIGNATURES = {
     "bh_read_csv": ([ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_U64), ctypes.c_char_p, ctypes.c_size_t], _I),
     "bh_free": ([_P], None),
     "bh_write_vtk": ([ctypes.c_char_p, _P, _U64, ctypes.c_char_p, ctypes.c_size_t], _I),
+    "bh_server_create": ([ctypes.POINTER(_P), _P, _U64, _U32], _I),
+    "bh_server_destroy": ([_P], None),
+    "bh_server_last_error": ([_P], ctypes.c_char_p),
+    "bh_server_footprint": ([_P], _U64),
+    "bh_server_submit": ([_P, _P, _P, _P, ctypes.POINTER(_U64)], _I),
+    "bh_server_step": ([_P, _P], _I),
+    "bh_server_poll": ([_P, _P, _U32, ctypes.POINTER(_U32)], _I),
+    "bh_server_pending": ([_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)], _I),
+    "bh_server_clock": ([_P], _D),
 }
 
 _lib = None
@@ -94,6 +104,37 @@ class StepStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ForceParamsC(ctypes.Structure):
+    _fields_ = [("theta", ctypes.c_double), ("eps2", ctypes.c_double), ("G", ctypes.c_double),
+                ("use_quadrupole", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("fp", ForceParamsC), ("max_leaf", ctypes.c_uint32),
+                ("group_size", ctypes.c_uint32), ("target_size", ctypes.c_uint32), ("steps", ctypes.c_uint32),
+                ("dt", ctypes.c_double)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("evaluations", ctypes.c_uint32), ("status", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("active_systems", ctypes.c_uint32), ("pending", ctypes.c_uint32), ("admitted", ctypes.c_uint32),
+                ("completed", ctypes.c_uint32), ("active_particles", ctypes.c_uint64),
+                ("stepped_particles", ctypes.c_uint64), ("used_bytes", ctypes.c_uint64), ("step_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def request(n, theta=0.5, eps2=0.0, G=1.0, use_quadrupole=True, max_leaf=16, group_size=256, target_size=32,
+            steps=1, dt=1e-3):
+    return Request(n, ForceParamsC(theta, eps2, G, 1 if use_quadrupole else 0, 0), max_leaf, group_size, target_size,
+                   steps, dt)
 
 
 def _ptr(a):
@@ -207,6 +248,12 @@ class Context:
                                             ctypes.byref(t)))
         return t
 
+    def state_step_batch(self, systems):
+        systems = np.ascontiguousarray(systems, dtype=SYSTEM)
+        t = StepTiming()
+        self._check(self._lib.bh_state_step_batch(self._h, _ptr(systems), len(systems), ctypes.byref(t)))
+        return t
+
     def state_download(self):
         out = np.zeros(self._n, PARTICLE)
         self._check(self._lib.bh_state_download(self._h, _ptr(out), len(out)))
@@ -285,3 +332,65 @@ class Context:
             acc = np.zeros((len(idx), 3))
             self._check(self._lib.bh_direct_sum(self._h, _ptr(p), len(p), _ptr(idx), len(idx), eps2, G, _ptr(acc)))
         return acc
+
+
+class Server:
+    """bh_server: continuous batching of independent leapfrog runs on one context."""
+
+    def __init__(self, ctx, mem_cap_bytes=200 * 10 ** 9, max_systems=65535):
+        self._lib = load()
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        rc = self._lib.bh_server_create(ctypes.byref(h), ctx._h, mem_cap_bytes, max_systems)
+        if rc != BH_OK:
+            raise BHError(rc, "bh_server_create failed")
+        self._h = h
+        self._results = {}  # ticket -> (result array, keep-alive)
+
+    def close(self):
+        if self._h:
+            self._lib.bh_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != BH_OK:
+            raise BHError(rc, self._lib.bh_server_last_error(self._h).decode())
+
+    def footprint(self, req):
+        return self._lib.bh_server_footprint(ctypes.byref(req))
+
+    def submit(self, particles, req):
+        """Queues a request; returns its ticket. The result array is handed
+        out by poll()."""
+        p = np.ascontiguousarray(particles, dtype=PARTICLE)
+        out = np.zeros(len(p), PARTICLE)
+        t = ctypes.c_uint64()
+        self._check(self._lib.bh_server_submit(self._h, _ptr(p), _ptr(out), ctypes.byref(req), ctypes.byref(t)))
+        self._results[t.value] = out
+        return t.value
+
+    def step(self):
+        st = ServerStats()
+        self._check(self._lib.bh_server_step(self._h, ctypes.byref(st)))
+        return st
+
+    def poll(self, max_items=4096):
+        """[(ticket, result particles, Completion)] completed since the last poll."""
+        buf = (Completion * max_items)()
+        cnt = ctypes.c_uint32()
+        self._check(self._lib.bh_server_poll(self._h, buf, max_items, ctypes.byref(cnt)))
+        return [(buf[i].ticket, self._results.pop(buf[i].ticket), buf[i]) for i in range(cnt.value)]
+
+    def pending(self):
+        p, a = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self._lib.bh_server_pending(self._h, ctypes.byref(p), ctypes.byref(a)))
+        return p.value, a.value
+
+    def clock(self):
+        return self._lib.bh_server_clock(self._h)

@@ -143,7 +143,6 @@ void release(DevBuf<T>& b) {
     if (rc_ != BH_OK) return rc_; \
   } while (0)
 
-constexpr uint64_t kMaxN = (1ull << 30) - 1;
 
 int check_args(Ctx* c, uint64_t n, const bh_force_params* fp, uint32_t max_leaf, uint32_t group_size) {
   if (n > kMaxN) {
@@ -467,6 +466,12 @@ std::vector<HostSys> host_systems(const bh_system* systems, uint32_t S) {
 
 Ctx* as(bh_ctx* c) { return reinterpret_cast<Ctx*>(c); }
 
+}  // namespace
+
+int bhk::ctx_device(const bh_ctx* c) { return c ? reinterpret_cast<const Ctx*>(c)->device : 0; }
+
+namespace {
+
 // Reference node numbering (flat_build_ttg.h:194-273): BFS backbone down to
 // subtree roots (count <= 4096 or leaf), then each subtree's other nodes BFS
 // in subtree-root order; group roots in push order (flat_tree.h:190-225).
@@ -715,6 +720,24 @@ int bh_state_step(bh_ctx* h, const bh_force_params* fp, uint32_t max_leaf, uint3
   bh_particle* in = c->cur == 0 ? c->pa.p : c->pb.p;
   bh_particle* out = c->cur == 0 ? c->pb.p : c->pa.p;
   TRY(step_device(c, in, out, c->state_n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing));
+  c->cur ^= 1;
+  return BH_OK;
+}
+
+int bh_state_step_batch(bh_ctx* h, const bh_system* systems, uint32_t n_systems, bh_step_timing* timing) {
+  Ctx* c = as(h);
+  if (!c || !systems) return c ? (set_err(c, "null argument"), BH_ERR_INVALID) : BH_ERR_INVALID;
+  const std::vector<HostSys> v = host_systems(systems, n_systems);
+  uint64_t n = 0;
+  for (const auto& q : v) n += q.n;
+  if (n != c->state_n) {
+    set_err(c, "the systems hold %llu particles, the resident state %llu", (unsigned long long)n,
+            (unsigned long long)c->state_n);
+    return BH_ERR_INVALID;
+  }
+  bh_particle* in = c->cur == 0 ? c->pa.p : c->pb.p;
+  bh_particle* out = c->cur == 0 ? c->pb.p : c->pa.p;
+  TRY(step_batch(c, in, out, v.data(), n_systems, timing));
   c->cur ^= 1;
   return BH_OK;
 }

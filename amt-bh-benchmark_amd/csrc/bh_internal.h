@@ -41,6 +41,10 @@ static_assert(sizeof(SysDev) == 64, "SysDev layout");
 
 constexpr uint32_t kChunk = 65536;   // particles per bounds block
 constexpr uint32_t kMaxSystems = 65535;
+constexpr uint64_t kMaxN = (1ull << 30) - 1;  // particles per step (all systems)
+
+// The device a context was created on.
+int ctx_device(const bh_ctx* ctx);
 
 // Device view of one tree (level-order BFS numbering).
 struct TreeDev {

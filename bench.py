@@ -87,6 +87,166 @@ def cpu_baseline(p, fp, args):
     }
 
 
+def ensemble_run(ctx, args):
+    """The mixed-N ensemble (64 systems, N log-uniform in [4096, 2^20], half
+    Plummer / half Hernquist, theta in {0.4, 0.5, 0.7}, eps 0.01): the
+    batched path (bh_step_batch, every system in the same launches) against
+    the plain path (one bh_step_device call per system), each over the
+    leapfrog schedule of `ensemble_steps` steps from the same resident start."""
+    import torch
+
+    from bhgpu import ensemble
+
+    t0 = time.time()
+    p, meta = ensemble.make()
+    n = len(p)
+    log(f"ensemble: {len(meta)} systems, {n} particles, generated in {time.time() - t0:.1f}s")
+    shapes = lambda m: (args.ens_max_leaf, args.ens_group_size, args.ens_target_size)  # noqa: E731
+    dt = 1e-3
+    sched = [(dt / 2, dt)] + [(dt, dt)] * (args.ensemble_steps - 1) + [(dt / 2, 0.0)]
+    tables = {kd: ensemble.table(meta, shapes, *kd) for kd in set(sched)}
+    offs = np.concatenate([[0], np.cumsum([m["n"] for m in meta])])
+    fps = [layout.force_params(m["theta"], m["eps2"], 1.0, True) for m in meta]
+    host = torch.from_numpy(p.view(np.uint8).copy())
+    buf = [torch.empty(n * 64, dtype=torch.uint8, device="cuda") for _ in range(2)]
+
+    def run(mode, schedule):
+        buf[0].copy_(host)
+        torch.cuda.synchronize()
+        cur, lat = 0, []
+        t_start = time.perf_counter()
+        for kick, drift in schedule:
+            s0 = time.perf_counter()
+            a, b = buf[cur].data_ptr(), buf[1 - cur].data_ptr()
+            if mode == "batched":
+                ctx.step_batch_device(a, b, tables[(kick, drift)])
+            else:
+                for k, m in enumerate(meta):
+                    o = int(offs[k]) * 64
+                    ml, gs, ts = shapes(m)
+                    ctx.step_device(a + o, b + o, m["n"], fps[k], ml, gs, ts, kick, drift)
+            lat.append(time.perf_counter() - s0)
+            cur ^= 1
+        wall = time.perf_counter() - t_start
+        return wall, lat, buf[cur].cpu().numpy().copy()
+
+    run("batched", sched[:1])  # grow the context's buffers
+    run("per_system", sched[:1])
+    res = {}
+    finals = {}
+    for mode in ("batched", "per_system"):
+        wall, lat, fin = run(mode, sched)
+        finals[mode] = fin
+        res[mode] = {
+            "particles_per_s": n * len(sched) / wall,
+            "ms_per_step": 1e3 * wall / len(sched),
+            "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        }
+        log(f"ensemble {mode}: {res[mode]}")
+    identical = bool(np.array_equal(finals["batched"], finals["per_system"]))
+
+    # accuracy of the batched step: a = v_out - v_in from one step with
+    # kick_dt = 1, drift_dt = 0, against the FP64 direct sum per system
+    kd = (1.0, 0.0)
+    buf[0].copy_(host)
+    torch.cuda.synchronize()
+    ctx.step_batch_device(buf[0].data_ptr(), buf[1].data_ptr(), ensemble.table(meta, shapes, *kd))
+    out = buf[1].cpu().numpy().view(layout.PARTICLE)
+    rng = np.random.default_rng(777)
+    errs, worst = [], 0.0
+    for k, m in enumerate(meta):
+        lo, hi = int(offs[k]), int(offs[k + 1])
+        ps, po = p[lo:hi], out[lo:hi]
+        acc = np.empty((hi - lo, 3))
+        acc[po["id"]] = po["vel"] - ps["vel"][po["id"]]
+        idx = np.sort(rng.choice(hi - lo, size=min(args.ens_error_subset, hi - lo), replace=False)).astype(np.uint32)
+        ref = ctx.direct_sum(ps, idx, m["eps2"], 1.0)
+        rel = np.linalg.norm(acc[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        errs.append(rel)
+        worst = max(worst, float(np.sqrt(np.mean(rel ** 2))))
+    rel = np.concatenate(errs)
+    return {
+        "workload": f"ensemble_{len(meta)}x_loguniform_4096_1048576",
+        "systems": len(meta),
+        "particles": int(n),
+        "evaluations": len(sched),
+        "shapes": {"max_leaf": args.ens_max_leaf, "group_size": args.ens_group_size,
+                   "target_size": args.ens_target_size},
+        "batched": res["batched"],
+        "per_system": res["per_system"],
+        "speedup": res["batched"]["particles_per_s"] / res["per_system"]["particles_per_s"],
+        "bit_identical_to_per_system": identical,
+        "accuracy": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "p99_rel": float(np.percentile(rel, 99)),
+                     "worst_system_rms_rel": worst, "subset_per_system": args.ens_error_subset,
+                     "reference": "FP64 direct sum"},
+    }
+
+
+def serving_run(ctx, args):
+    """A prefix of the serving trace: requests with Poisson arrivals at
+    `serve_rate`/s, N log-uniform in [1024, 2^21] Plummer systems (request k
+    from seed 5000 + k), 1-50 leapfrog steps, theta 0.5, eps 0.01, replayed
+    against the server clock through the continuous-batching server
+    (admission by footprint under `serve_cap_gb`). Latency runs from a
+    request's scheduled arrival to its completion."""
+    R = args.serve_requests
+    st = ic.stream_state(4242, np.arange(R, dtype=np.uint64))
+    arrivals = np.cumsum(-np.log1p(-ic.uniform(st, 0)) / args.serve_rate)
+    ns = np.rint(1024 * 2048.0 ** ic.uniform(st, 1)).astype(np.int64)
+    steps = 1 + np.floor(50 * ic.uniform(st, 2)).astype(np.int64)
+    t0 = time.time()
+    parts = [ic.plummer(int(n), 5000 + k) for k, n in enumerate(ns)]
+    log(f"serving: {R} requests, {int(ns.sum())} particles, generated in {time.time() - t0:.1f}s")
+    reqs = [bl.request(int(n), 0.5, 1e-4, True, args.ens_max_leaf, args.ens_group_size, args.ens_target_size,
+                       int(k), 1e-3) for n, k in zip(ns, steps)]
+    srv = bl.Server(ctx, mem_cap_bytes=int(args.serve_cap_gb * 1e9))
+    # warm the context's buffers with the largest request alone
+    big = int(np.argmax(ns))
+    srv.submit(parts[big], bl.request(int(ns[big]), 0.5, 1e-4, True, args.ens_max_leaf, args.ens_group_size,
+                                      args.ens_target_size, 1, 1e-3))
+    while srv.pending() != (0, 0):
+        srv.step()
+    srv.poll()
+    ticket_of = {}
+    lat, batch_sizes, batch_parts, step_ms, used = [], [], [], [], []
+    i, done = 0, 0
+    start = srv.clock()
+    while done < R:
+        now = srv.clock() - start
+        while i < R and arrivals[i] <= now:
+            ticket_of[srv.submit(parts[i], reqs[i])] = i
+            i += 1
+        if srv.pending() == (0, 0):
+            time.sleep(max(0.0, min(arrivals[i] - (srv.clock() - start), 0.05)))
+            continue
+        stt = srv.step()
+        if stt.stepped_particles:
+            batch_sizes.append(stt.active_systems + stt.completed)
+            batch_parts.append(stt.stepped_particles)
+            step_ms.append(stt.step_ms)
+            used.append(stt.used_bytes)
+        for t, _, c in srv.poll():
+            k = ticket_of.pop(t)
+            lat.append(c.done_s - (start + arrivals[k]))
+            done += 1
+    makespan = srv.clock() - start
+    srv.close()
+    work = float((ns * (steps + 1)).sum())
+    return {
+        "workload": f"serving_trace_{R}req_poisson{args.serve_rate:g}_loguniform_1024_2097152",
+        "requests": R,
+        "particle_evaluations": work,
+        "makespan_s": makespan,
+        "particles_per_s": work / makespan,
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(np.max(lat))},
+        "batch": {"steps": len(step_ms), "systems_mean": float(np.mean(batch_sizes)),
+                  "particles_mean": float(np.mean(batch_parts)), "step_ms_p50": float(np.percentile(step_ms, 50)),
+                  "step_ms_p99": float(np.percentile(step_ms, 99)), "peak_footprint_gb": max(used) / 1e9},
+        "admission_cap_gb": args.serve_cap_gb,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +264,17 @@ def main():
     ap.add_argument("--error-subset", type=int, default=4096)
     ap.add_argument("--cpu-stride", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the mixed-N ensemble comparison")
+    ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--ens-max-leaf", type=int, default=16)
+    ap.add_argument("--ens-group-size", type=int, default=256)
+    ap.add_argument("--ens-target-size", type=int, default=32)
+    ap.add_argument("--ens-error-subset", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=200, help="serving-trace prefix length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-cap-gb", type=float, default=200.0)
+    ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (torch) three times, to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -127,6 +298,17 @@ def main():
         print("--steps and --warmup must be >= 1 (the first warm-up step is the leapfrog's half-kick prime)",
               file=sys.stderr)
         return 2
+
+    if args.serving_only:
+        ctx = bl.Context(0)
+        print(json.dumps(serving_run(ctx, args)), flush=True)
+        ctx.close()
+        return 0
+    if args.ensemble_only:
+        ctx = bl.Context(0)
+        print(json.dumps(ensemble_run(ctx, args)), flush=True)
+        ctx.close()
+        return 0
 
     t0 = time.time()
     p = ic.plummer(args.n, args.seed)
@@ -200,7 +382,16 @@ def main():
             "traffic": None,
         },
     }
+    if not args.no_ensemble:
+        del p
+        out["ensemble"] = ensemble_run(ctx, args)
+        p = None
+    if args.serve_requests > 0:
+        out["serving"] = serving_run(ctx, args)
+        log(f"serving: {out['serving']}")
     if not args.no_cpu_baseline:
+        if p is None:
+            p = ic.plummer(args.n, args.seed)
         out["cpu_baseline"] = cpu_baseline(p, fp, args)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)

@@ -140,11 +140,66 @@ int bh_step_batch(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, cons
 int bh_step_batch_host(bh_ctx* ctx, bh_particle* particles, const bh_system* systems, uint32_t n_systems,
                        bh_step_timing* timing);
 
+/* ---- Continuous-batching server (no reference counterpart) ----
+ * Many independent leapfrog runs share the GPU: each server step admits
+ * queued requests (in submission order) while the active systems'
+ * footprints stay within the memory cap, runs one force evaluation of every
+ * active system in one bh_step_batch, and retires the systems whose
+ * schedule is complete. A request runs main.cc:199-246's schedule: (dt/2,
+ * dt), steps - 1 times (dt, dt), then (dt/2, 0); its result (Morton order)
+ * is bit-identical to running that schedule alone with bh_step. One thread
+ * per server; the server steps on its context's stream. */
+typedef struct bh_server bh_server;
+typedef struct bh_request {
+  uint64_t n;
+  bh_force_params fp;
+  uint32_t max_leaf;
+  uint32_t group_size;
+  uint32_t target_size;
+  uint32_t steps; /* leapfrog steps: steps + 1 force evaluations */
+  double dt;
+} bh_request;
+typedef struct bh_completion {
+  uint64_t ticket;
+  double submit_s, admit_s, done_s; /* server clock, seconds */
+  uint32_t evaluations;
+  int32_t status;
+} bh_completion;
+typedef struct bh_server_stats {
+  uint32_t active_systems; /* after the step */
+  uint32_t pending;
+  uint32_t admitted;
+  uint32_t completed;
+  uint64_t active_particles;
+  uint64_t stepped_particles; /* particles evaluated by this step */
+  uint64_t used_bytes;        /* footprint of the active systems */
+  double step_ms;
+} bh_server_stats;
+
+int bh_server_create(bh_server** out, bh_ctx* ctx, uint64_t mem_cap_bytes, uint32_t max_systems);
+void bh_server_destroy(bh_server* server);
+const char* bh_server_last_error(const bh_server* server);
+/* Device bytes a request is counted for against the cap. */
+uint64_t bh_server_footprint(const bh_request* request);
+/* Queues a request; `particles` is copied at once, `result` (n particles)
+ * must stay valid until the request's completion is polled. */
+int bh_server_submit(bh_server* server, const bh_particle* particles, bh_particle* result, const bh_request* request,
+                     uint64_t* ticket);
+/* One admission + batched evaluation + retirement round. */
+int bh_server_step(bh_server* server, bh_server_stats* stats);
+/* Completed requests since the last poll, oldest first. */
+int bh_server_poll(bh_server* server, bh_completion* out, uint32_t max, uint32_t* count);
+int bh_server_pending(const bh_server* server, uint32_t* pending, uint32_t* active);
+double bh_server_clock(const bh_server* server);
+
 /* Particles resident in device memory across steps (the caller's
  * GroupedWorkspace reuse, main.cc:203-242, without host copies). */
 int bh_state_upload(bh_ctx* ctx, const bh_particle* host, uint64_t n);
 int bh_state_step(bh_ctx* ctx, const bh_force_params* fp, uint32_t max_leaf, uint32_t group_size, uint32_t target_size,
                   double kick_dt, double drift_dt, bh_step_timing* timing);
+/* One bh_step_batch over the resident particles (the systems' sizes add up
+ * to the resident count). */
+int bh_state_step_batch(bh_ctx* ctx, const bh_system* systems, uint32_t n_systems, bh_step_timing* timing);
 int bh_state_download(bh_ctx* ctx, bh_particle* host, uint64_t n);
 /* Device address of the resident particles (Morton order after a step). */
 bh_particle* bh_state_device_ptr(bh_ctx* ctx);

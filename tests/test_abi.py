@@ -39,3 +39,30 @@ def test_records_match_header():
     assert layout.PARTICLE.itemsize == 64 and layout.PARTICLE.fields["pos"][1] == 16
     assert layout.WALK_NODE.itemsize == 56 and layout.WALK_NODE.fields["first_child"][1] == 40
     assert layout.FORCE_PARAMS.itemsize == 32
+
+
+def test_struct_layouts_match_c(tmp_path):
+    """The Python records have the C compiler's sizes and offsets."""
+    import shutil
+    import subprocess
+
+    from bhgpu import layout
+
+    if not shutil.which("gcc"):
+        pytest.skip("no gcc")
+    inc = os.path.dirname(HEADER)
+    src = tmp_path / "sizes.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "bh_gpu.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bh_particle), sizeof(bh_force_params),\n'
+        "         sizeof(bh_system), offsetof(bh_system, kick_dt), sizeof(bh_request), offsetof(bh_request, dt),\n"
+        "         sizeof(bh_completion), sizeof(bh_server_stats), sizeof(bh_walk_node));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c11", "-I", inc, "-o", str(exe), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [layout.PARTICLE.itemsize, layout.FORCE_PARAMS.itemsize, layout.SYSTEM.itemsize,
+            layout.SYSTEM.fields["kick_dt"][1], ctypes.sizeof(bl.Request), bl.Request.dt.offset,
+            ctypes.sizeof(bl.Completion), ctypes.sizeof(bl.ServerStats), layout.WALK_NODE.itemsize]
+    assert got == want

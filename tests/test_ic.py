@@ -107,3 +107,30 @@ def test_hernquist_deterministic_prefix_stable():
     b = ic.hernquist(800, 9)
     assert np.array_equal(a["pos"], b["pos"][:500]) and np.array_equal(a["vel"], b["vel"][:500])
     assert np.array_equal(a, ic.hernquist(500, 9))
+
+
+def test_hernquist_rejection_bounds_hold():
+    """Both rejection regimes are exact: b(q) <= b(1) = 3 pi / 2 on [0, 1]
+    (inner proposal), and the tabulated bound dominates g(s) on a fine grid
+    for radii in [1, 20) (outer)."""
+    q = np.linspace(0.0, 1.0, 100001)
+    b = ic._hernquist_b(q)
+    assert np.all(b <= 1.5 * np.pi + 1e-12) and abs(b[-1] - 1.5 * np.pi) < 1e-12
+    assert np.all(np.diff(b) >= -1e-15)  # f grows with E
+    r = np.geomspace(1.0, 19.999, 500)
+    psi = 1.0 / (1.0 + r)
+    s = np.linspace(0.0, 1.0, 4001)
+    g = (s * s * ic.hernquist_df(psi[:, None] * (1.0 - s * s))).max(axis=1)
+    bound = ic.hernquist_bound(r)
+    assert np.all(g <= bound) and np.all(bound < 2.0 * g)
+
+
+def test_hernquist_inner_proposal_inverts_cdf():
+    """s = t sqrt(r / (1 - t^2)) maps x uniform to density ~ s^2 (r + s^2)^(-5/2)
+    on [0, 1]: its CDF s^3 / (r + s^2)^(3/2) * (1 + r)^(3/2) equals x."""
+    for r in (1e-6, 0.01, 0.5, 0.999):
+        x = np.linspace(0.0, 1.0, 11)
+        t = np.cbrt(x) / np.sqrt(1.0 + r)
+        s = t * np.sqrt(r / (1.0 - t * t))
+        assert np.allclose(s ** 3 / (r + s * s) ** 1.5 * (1 + r) ** 1.5, x, atol=1e-12)
+        assert abs(s[-1] - 1.0) < 1e-9  # 1 - t^2 cancels at small r
