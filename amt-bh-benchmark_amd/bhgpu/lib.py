@@ -25,6 +25,7 @@ STATUS = {
     7: "BH_ERR_NO_TREE",
 }
 BH_FP32, BH_FP64 = 0, 1
+BH_WALK_GROUPED, BH_WALK_PER_TARGET = 0, 1
 BH_LAYOUT_REFERENCE, BH_LAYOUT_BFS = 0, 1
 
 # Every entry point of include/bh_gpu.h with its ctypes signature.
@@ -38,6 +39,7 @@ SIGNATURES = {
     "bh_ctx_destroy": ([_P], None),
     "bh_last_error": ([_P], ctypes.c_char_p),
     "bh_set_precision": ([_P, _I], _I),
+    "bh_set_walk_mode": ([_P, _I], _I),
     "bh_max_particles": ([], _U64),
     "bh_step": ([_P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_step_device": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
@@ -124,7 +126,8 @@ class Completion(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("active_systems", ctypes.c_uint32), ("pending", ctypes.c_uint32), ("admitted", ctypes.c_uint32),
-                ("completed", ctypes.c_uint32), ("active_particles", ctypes.c_uint64),
+                ("completed", ctypes.c_uint32), ("stepped_systems", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("active_particles", ctypes.c_uint64),
                 ("stepped_particles", ctypes.c_uint64), ("used_bytes", ctypes.c_uint64), ("step_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -205,6 +208,9 @@ class Context:
 
     def set_precision(self, precision):
         self._check(self._lib.bh_set_precision(self._h, precision))
+
+    def set_walk_mode(self, mode):
+        self._check(self._lib.bh_set_walk_mode(self._h, mode))
 
     def step(self, p, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
         """step_particles: returns (particles in Morton order, StepTiming)."""

@@ -38,6 +38,7 @@ struct Ctx {
   bool have_stats = false;
   std::string err;
   int precision = BH_FP32;
+  int walk_mode = BH_WALK_GROUPED;
 
   DevBuf<bh_particle> pa, pb;   // particle ping-pong (resident state / step output)
   int cur = 0;                  // which of pa/pb holds the resident state
@@ -52,7 +53,10 @@ struct Ctx {
   DevBuf<double4> posm;
   DevBuf<uint8_t> L, H;
   DevBuf<uint32_t> tcounts, tprefix;
-  DevBuf<uint32_t> mark, targets;
+  DevBuf<uint32_t> mark, targets;   // walk targets: marks per particle, list
+  DevBuf<uint32_t> gmark, groups;   // walk groups: marks per particle, list
+  DevBuf<uint32_t> defer;           // deferred nodes per group
+  DevBuf<double> gacc;              // group sums per particle
   // tree (level-order BFS)
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
   DevBuf<uint8_t> depth, leaf, nchild;
@@ -383,14 +387,22 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   TRY(ensure(c, c->mark, n, "targets"));
   TRY(ensure(c, c->targets, n, "targets"));
+  TRY(ensure(c, c->gmark, n, "groups"));
+  TRY(ensure(c, c->groups, n, "groups"));
+  TRY(ensure(c, c->gacc, 3ull * n, "group sums"));
   TRY(ensure(c, c->totals, 2, "totals"));
-  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), c->mark.p, c->tcounts.p, c->tprefix.p,
-                           c->targets.p, c->misc.p + 40, s));
+  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), false, c->mark.p, c->tcounts.p,
+                           c->tprefix.p, c->targets.p, c->misc.p + 40, s));
+  // per-target mode: the walk groups are the targets themselves
+  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), c->walk_mode == BH_WALK_GROUPED,
+                           c->gmark.p, c->tcounts.p, c->tprefix.p, c->groups.p, c->misc.p + 41, s));
   HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
   HIP_OK(c, hipEventRecord(c->ev[3], s));
-  HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_OK(c, hipStreamSynchronize(s));
   (void)tiles;
+  const uint32_t n_groups = c->host_misc[41];
+  TRY(ensure(c, c->defer, uint64_t(n_groups) * kDefer, "deferred nodes"));
   WalkArgs a;
   a.mac = c->mac.p;
   a.link = c->link.p;
@@ -400,10 +412,13 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   a.begin = c->begin.p;
   a.count = c->count.p;
   a.posm = c->posm.p;
-  a.targets = c->targets.p;
+  a.groups = c->groups.p;
+  a.tmark = c->mark.p;
   a.sid = c->sid.p;
   a.sys = sys_dev(c);
-  a.n_targets = c->host_misc[40];
+  a.gacc = c->gacc.p;
+  a.defer = c->defer.p;
+  a.n_groups = n_groups;
   a.n_nodes = c->n_nodes;
   a.n_parts = c->n;
   a.parts = parts;
@@ -578,6 +593,10 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->tprefix);
   release(c->mark);
   release(c->targets);
+  release(c->gmark);
+  release(c->groups);
+  release(c->defer);
+  release(c->gacc);
   release(c->begin);
   release(c->count);
   release(c->cs);
@@ -625,6 +644,17 @@ int bh_set_precision(bh_ctx* h, int precision) {
     return BH_ERR_INVALID;
   }
   c->precision = precision;
+  return BH_OK;
+}
+
+int bh_set_walk_mode(bh_ctx* h, int mode) {
+  Ctx* c = as(h);
+  if (!c) return BH_ERR_INVALID;
+  if (mode != BH_WALK_GROUPED && mode != BH_WALK_PER_TARGET) {
+    set_err(c, "unknown walk mode %d", mode);
+    return BH_ERR_INVALID;
+  }
+  c->walk_mode = mode;
   return BH_OK;
 }
 

@@ -42,6 +42,8 @@ static_assert(sizeof(SysDev) == 64, "SysDev layout");
 constexpr uint32_t kChunk = 65536;   // particles per bounds block
 constexpr uint32_t kMaxSystems = 65535;
 constexpr uint64_t kMaxN = (1ull << 30) - 1;  // particles per step (all systems)
+constexpr uint32_t kGroupCap = 256;  // walk groups: topmost nodes with leaf || count <= max(tsm, kGroupCap)
+constexpr uint32_t kDefer = 512;     // deferred nodes a group may hand its targets
 
 // The device a context was created on.
 int ctx_device(const bh_ctx* ctx);
@@ -79,10 +81,13 @@ struct WalkArgs {
   const uint32_t* begin;
   const uint32_t* count;
   const double4* posm;     // x, y, z, m in Morton order
-  const uint32_t* targets; // target nodes, Morton order
+  const uint32_t* groups;  // walk groups (nodes), Morton order
+  const uint32_t* tmark;   // [n] target node + 1 at each target's first particle, else 0
   const uint16_t* sid;     // system of each particle
   const SysDev* sys;       // system table; the root of system s is node s
-  uint32_t n_targets;
+  double* gacc;            // [3n] scratch: a group's shared-list sums per particle
+  uint32_t* defer;         // [n_groups * kDefer] scratch: deferred nodes per group
+  uint32_t n_groups;
   uint32_t n_nodes;        // index clamps: a node or particle index never
   uint32_t n_parts;        // leaves its array, whatever the tree holds
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
@@ -121,9 +126,12 @@ hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, cons
                             const uint32_t* totals, TreeDev t, hipStream_t s);
 hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid,
                               const double* lohi, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s);
-hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, uint32_t* mark,
-                          uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets,
-                          hipStream_t s);
+// Marks (node + 1 at its first particle) and lists the topmost nodes with
+// leaf || count <= thr: thr = the system's tsm (walk targets) or, with
+// groups set, max(tsm, kGroupCap) (walk groups).
+hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
+                          uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
+                          uint32_t* n_targets, hipStream_t s);
 
 // ---- bh_walk.hip
 hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s);

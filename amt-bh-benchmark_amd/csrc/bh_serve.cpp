@@ -294,6 +294,7 @@ int bh_server_step(bh_server* h, bh_server_stats* stats) {
   }
   uint64_t stepped = 0;
   for (const auto& a : s->active) stepped += a.q.r.n;
+  const uint32_t stepped_systems = static_cast<uint32_t>(s->active.size());
 
   // retire finished systems, pack the rest (then the new ones) into `in`
   std::vector<Active> keep;
@@ -334,6 +335,7 @@ int bh_server_step(bh_server* h, bh_server_stats* stats) {
     stats->active_systems = static_cast<uint32_t>(s->active.size());
     stats->active_particles = active_particles(s);
     stats->stepped_particles = stepped;
+    stats->stepped_systems = stepped_systems;
     stats->completed = completed;
     stats->step_ms = 1e3 * (t_end - t_step);
     stats->used_bytes = s->used;

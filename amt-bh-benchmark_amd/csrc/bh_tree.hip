@@ -353,9 +353,10 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
 // leaf || count <= tsm (tsm = min(target_size, group_size), which makes the
 // group-root / target nesting of grouped_ttg.h:251-288 a single predicate).
 __global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_nodes, const uint16_t* __restrict__ sid,
-                                                        const SysDev* __restrict__ sys, uint32_t* __restrict__ mark) {
+                                                        const SysDev* __restrict__ sys, uint32_t floor_thr,
+                                                        uint32_t* __restrict__ mark) {
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
-    const uint32_t tsm = sys[sid[t.begin[i]]].tsm;
+    const uint32_t tsm = max(sys[sid[t.begin[i]]].tsm, floor_thr);
     const bool pred = t.leaf[i] || t.count[i] <= tsm;
     if (!pred) continue;
     const bool ppred = (t.depth[i] != 0) && (t.count[t.parent[i]] <= tsm);
@@ -446,13 +447,14 @@ hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, uint32_t* mark,
-                          uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets,
-                          hipStream_t s) {
+hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
+                          uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
+                          uint32_t* n_targets, hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   hipError_t e = launch_fill_u32(mark, 0u, n, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys, mark);
+  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys,
+                     groups ? kGroupCap : 0u, mark);
   hipLaunchKernelGGL(k_compact_count, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_counts);
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, 1u, tile_prefix, n_targets);
   hipLaunchKernelGGL(k_compact_write, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_prefix, targets);

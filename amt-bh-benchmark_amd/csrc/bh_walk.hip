@@ -1,4 +1,4 @@
-// The grouped force walk on CDNA4: one wavefront per walk target.
+// The grouped force walk on CDNA4: one wavefront per walk group.
 //
 // Reference semantics:
 //   targets           grouped_ttg.h:253 (topmost nodes with leaf || count <= target_size)
@@ -12,25 +12,34 @@
 //   P2C               multipole.h:171-200: monopole + traceless quadrupole
 //   kick/drift        integrate.h:15-16 via integrate_ttg.h:119-122, fused
 //
-// Each target walks the tree from the root on its own. That gives exactly
-// the interaction list of the reference's shared group walk: its shared
-// accept / open-for-all decisions at a group node imply the same decision
-// for every target box inside that node's box (near2 and far2 are monotone
-// in the box, also in floating point), and a target opens whatever it does
-// not accept (grouped_ttg.h:42-51).
+// Work is split into walk groups: topmost nodes with leaf || count <=
+// max(tsm, 256), one wavefront each. A group that is itself a target walks
+// as one target. Otherwise the wavefront first walks the tree for the
+// group's particle box with the reference's three-way decision at a group
+// node (grouped_ttg.h:259-276): accepted (theta^2 near2 > edge^2) -> the
+// shared list, rejected for every box inside (theta^2 far2 <= edge^2) ->
+// opened for all, otherwise deferred. The shared list is evaluated for all
+// the group's particles at once (each lane holds up to 4 of them). Then
+// each target of the group walks only the deferred nodes. This gives every
+// target exactly the interaction list of its own walk from the root: near2
+// and far2 are monotone in the box (also in floating point) and near2 <=
+// far2, so a group's accept / reject-for-all is the decision of every
+// target box inside, and a target opens whatever it does not accept
+// (grouped_ttg.h:42-51).
 //
 // Per iteration a wavefront pops up to 64 nodes of the top frame of its
-// LDS stack (frame d holds child ranges at tree depth d), tests them with
-// 64 lanes in FP64 - bit-identical to the reference's decisions - and
-// compacts the outcomes with ballots: children ranges into frame d+1,
-// accepted cells into the P2C ring, opened leaves' particles into the P2P
-// ring. Ring entries are staged once per target as (source - target centre)
-// in FP64, then rounded to the walk precision, so FP32 arithmetic never sees
-// absolute coordinates. Whenever a ring holds 64 entries they are evaluated:
-// lane = (target particle p, slice s); with T particles per target each of
-// the S = 64 / T slices takes every S-th entry, so small leaves still use
-// the whole wavefront. FP32 batches are accumulated per lane with Kahan
-// compensation; slices are combined in FP64.
+// LDS stack (frame d holds child ranges d levels below the walk's start),
+// tests them with 64 lanes in FP64 - bit-identical to the reference's
+// decisions - and compacts the outcomes with ballots: children ranges into
+// frame d+1, accepted cells into the P2C ring, opened leaves' particles
+// into the P2P ring. Ring entries are staged as (source - box centre) in
+// FP64, then rounded to the walk precision, so FP32 arithmetic never sees
+// absolute coordinates. Whenever a ring holds a batch it is evaluated. For a
+// target with T particles, lane = (particle p, slice s): each of the
+// S = 64 / T slices takes every S-th entry, so small leaves still use the
+// whole wavefront. FP32 batches are accumulated per lane with Kahan
+// compensation; slices, the group's share and the target's own list are
+// combined in FP64.
 //
 // No MFMA: the walk is not a dense contraction.
 #include "bh_internal.h"
@@ -40,6 +49,7 @@ namespace bhk {
 namespace {
 
 constexpr int kBuf = 128;  // staging capacity (entries); evaluated in batches of <= 64
+constexpr int kGroupChunks = kGroupCap / kWave;  // a group's particles per lane
 
 template <typename Real>
 struct Vec4T;
@@ -210,16 +220,383 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// One target (node tnode): the full walk for each chunk of <= 64 of its
-// particles, then reduction, output and kick/drift.
-//
-// Evaluation layout: with tc particles in the chunk, S = 64 / tc slices and
-// C = 64 / S entries per slice, a batch is B = S * C staged entries; lane
-// (p, s) evaluates entries [s*C, (s+1)*C) of the batch for particle p, so the
-// loop trip count is uniform and the LDS offsets are per-lane constants.
-template <typename Real, bool kMask>
-__device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode) {
+// Group-phase P2P / P2C: every lane holds NC (<= 4) particles of the group,
+// lane + 64 k, and takes all 64 entries of the batch (LDS broadcast reads).
+template <typename Real, bool kMask, int NC>
+__device__ inline void eval_pp_group(const typename Vec4T<Real>::T* e, const Real* gx, const Real* gy, const Real* gz,
+                                     Real eps2, Acc<Real>* acc) {
   using V4 = typename Vec4T<Real>::T;
+  Real bx[NC], by[NC], bz[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
+#pragma unroll 2
+  for (int j = 0; j < kWave; ++j) {
+    const V4 q = e[j];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const Real dx = q.x - gx[k], dy = q.y - gy[k], dz = q.z - gz[k];
+      const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
+      Real inv = rsq(r2);
+      if (kMask) inv = r2 > Real(0) ? inv : Real(0);
+      const Real s3 = q.w * inv * inv * inv;
+      bx[k] = fmaR(s3, dx, bx[k]);
+      by[k] = fmaR(s3, dy, by[k]);
+      bz[k] = fmaR(s3, dz, bz[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) acc[k].add(bx[k], by[k], bz[k]);
+}
+
+template <typename Real, int NC>
+__device__ inline void eval_pc_group(const typename Vec4T<Real>::T* e, const Real* gx, const Real* gy, const Real* gz,
+                                     Real eps2, Acc<Real>* acc) {
+  using V4 = typename Vec4T<Real>::T;
+  Real bx[NC], by[NC], bz[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
+  for (int j = 0; j < kWave; ++j) {
+    const V4 e0 = e[3 * j + 0];
+    const V4 e1 = e[3 * j + 1];
+    const V4 e2 = e[3 * j + 2];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const Real dx = e0.x - gx[k], dy = e0.y - gy[k], dz = e0.z - gz[k];
+      const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
+      const Real inv = rsq(r2);
+      const Real inv2 = inv * inv;
+      const Real inv3 = inv * inv2;
+      const Real qx = fmaR(e1.x, dx, fmaR(e1.w, dy, e2.x * dz));
+      const Real qy = fmaR(e1.w, dx, fmaR(e1.y, dy, e2.y * dz));
+      const Real qz = fmaR(e2.x, dx, fmaR(e2.y, dy, e1.z * dz));
+      const Real dqd = fmaR(dx, qx, fmaR(dy, qy, dz * qz));
+      const Real inv5 = inv3 * inv2;
+      const Real inv7 = inv5 * inv2;
+      const Real f = fmaR(Real(2.5) * dqd, inv7, e0.w * inv3);
+      bx[k] = fmaR(f, dx, fmaR(-inv5, qx, bx[k]));
+      by[k] = fmaR(f, dy, fmaR(-inv5, qy, by[k]));
+      bz[k] = fmaR(f, dz, fmaR(-inv5, qz, bz[k]));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) acc[k].add(bx[k], by[k], bz[k]);
+}
+
+// Evaluator of a target's particles: lane = (particle p, slice s), batches
+// of B = S * C entries, slice s takes entries [s*C, (s+1)*C).
+template <typename Real, bool kMask>
+struct TargetEval {
+  using V4 = typename Vec4T<Real>::T;
+  Real tx, ty, tz, eps2;
+  int C, s;
+  bool active;
+  uint32_t B;
+  Acc<Real> acc;
+  __device__ inline void pp(const V4* ring) {
+    if (active) eval_pp<Real, kMask>(&ring[s * C], C, tx, ty, tz, eps2, acc);
+  }
+  __device__ inline void pc(const V4* ring) {
+    if (active) eval_pc<Real>(&ring[3 * s * C], C, tx, ty, tz, eps2, acc);
+  }
+};
+
+// Evaluator of a whole group (<= 256 particles): lane + 64 k, batches of 64.
+template <typename Real, bool kMask>
+struct GroupEval {
+  using V4 = typename Vec4T<Real>::T;
+  Real gx[kGroupChunks], gy[kGroupChunks], gz[kGroupChunks], eps2;
+  int nc;
+  uint32_t B;
+  Acc<Real> acc[kGroupChunks];
+  __device__ inline void pp(const V4* ring) {
+    switch (nc) {
+      case 1: eval_pp_group<Real, kMask, 1>(ring, gx, gy, gz, eps2, acc); break;
+      case 2: eval_pp_group<Real, kMask, 2>(ring, gx, gy, gz, eps2, acc); break;
+      case 3: eval_pp_group<Real, kMask, 3>(ring, gx, gy, gz, eps2, acc); break;
+      default: eval_pp_group<Real, kMask, 4>(ring, gx, gy, gz, eps2, acc); break;
+    }
+  }
+  __device__ inline void pc(const V4* ring) {
+    switch (nc) {
+      case 1: eval_pc_group<Real, 1>(ring, gx, gy, gz, eps2, acc); break;
+      case 2: eval_pc_group<Real, 2>(ring, gx, gy, gz, eps2, acc); break;
+      case 3: eval_pc_group<Real, 3>(ring, gx, gy, gz, eps2, acc); break;
+      default: eval_pc_group<Real, 4>(ring, gx, gy, gz, eps2, acc); break;
+    }
+  }
+};
+
+// The depth-first walk of one box (a target's or a group's particles) over
+// the node ranges `start` (global, packed (first << 3) | (count - 1)), or
+// from the system's root when start is null. Accepted nodes and the
+// particles of opened leaves are staged in the rings relative to (cx, cy,
+// cz) and handed to ev in batches of ev.B; the last partial batch is padded.
+//
+// kGroup: a group's walk (grouped_ttg.h:259-276 at the group node): a node
+// that is neither accepted nor rejected for every box inside
+// (theta^2 * far2 > edge^2) is deferred: appended to dl, for its targets.
+// Returns false when more than kDefer nodes would be deferred.
+template <typename Real, bool kMask, bool kGroup, typename Eval>
+__device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box, double cx, double cy, double cz,
+                         uint32_t sys_id, const uint32_t* start, uint32_t n_start, bool use_quad, double theta2,
+                         Eval& ev, uint32_t& n_pp, uint32_t& n_pc, uint32_t* dl, uint32_t& n_def) {
+  using V4 = typename Vec4T<Real>::T;
+  const int lane = __lane_id();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t last_node = a.n_nodes - 1, last_part = a.n_parts - 1;
+  const uint32_t B = ev.B;
+  const double blo0 = box[0], blo1 = box[1], blo2 = box[2], bhi0 = box[3], bhi1 = box[4], bhi2 = box[5];
+  uint32_t pp_cnt = 0, pc_cnt = 0;
+  bool ok = true;
+
+  // Evaluate the first B staged entries, then move the rest (< 64: at most
+  // B - 1 + 64 were staged) to the front; callers repeat while >= B remain.
+  auto flush_pp = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    ev.pp(L.pp);
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t rest = pp_cnt - B;
+    V4 v;
+    if (static_cast<uint32_t>(lane) < rest) v = L.pp[B + lane];
+    __builtin_amdgcn_wave_barrier();
+    if (static_cast<uint32_t>(lane) < rest) L.pp[lane] = v;
+    pp_cnt = rest;
+    __builtin_amdgcn_wave_barrier();
+  };
+  auto flush_pc = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    ev.pc(L.pc);
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t rest = pc_cnt - B;
+    V4 v0, v1, v2;
+    if (static_cast<uint32_t>(lane) < rest) {
+      v0 = L.pc[3 * (B + lane) + 0];
+      v1 = L.pc[3 * (B + lane) + 1];
+      v2 = L.pc[3 * (B + lane) + 2];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (static_cast<uint32_t>(lane) < rest) {
+      L.pc[3 * lane + 0] = v0;
+      L.pc[3 * lane + 1] = v1;
+      L.pc[3 * lane + 2] = v2;
+    }
+    pc_cnt = rest;
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  if (lane == 0) {
+    L.fr[0][0] = sys_id << 3;  // the system's root: node sys_id, one node
+    L.fr_n[0] = start ? n_start : 1u;
+    L.fr_cur[0] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  int depth = 0;
+  while (depth >= 0) {
+    const uint32_t nr = uniform(L.fr_n[depth]);
+    const uint32_t cur = uniform(L.fr_cur[depth]);
+    if (cur >= nr) {
+      --depth;
+      continue;
+    }
+    // Pop whole child ranges, up to 64 nodes. Ranges of a frame are
+    // ascending and disjoint, so first - excl is non-decreasing over the
+    // ranges and a prefix max of it, placed at each range's first slot,
+    // yields every slot's node: node = (first_j - excl_j) + slot.
+    const uint32_t rr = cur + lane;
+    const bool rv = rr < nr;
+    uint32_t node, total;
+    if (depth == 0 && start) {
+      // a deferred list: single nodes in walk order, one per lane
+      node = rv ? (start[rr] >> 3) : 0u;
+      total = nr - cur < kWave ? nr - cur : kWave;
+      if (lane == 0) L.fr_cur[0] = cur + total;
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      const uint32_t packed = rv ? L.fr[depth][rr] : 0u;
+      const uint32_t rcnt = rv ? (packed & 7u) + 1u : 0u;
+      const uint32_t rfirst = packed >> 3;
+      const uint32_t incl = wave_incl_add(rcnt);
+      const bool taken = rv && incl <= kWave;
+      const int k = __popcll(__ballot(taken));
+      total = uniform(__builtin_amdgcn_readlane(incl, k - 1));
+      const uint32_t excl = incl - rcnt;
+      L.own[lane] = 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (taken) L.own[excl] = rfirst - excl;
+      if (lane == 0) L.fr_cur[depth] = cur + k;
+      __builtin_amdgcn_wave_barrier();
+      node = wave_incl_max(L.own[lane]) + static_cast<uint32_t>(lane);
+    }
+    const bool nv = static_cast<uint32_t>(lane) < total;
+    // MAC in FP64, reference operation order (multipole.h:130-160)
+    bool accept = false, isleaf = false, defer = false;
+    uint32_t lfirst = 0, lcnt = 0;
+    double4 mac = make_double4(0, 0, 0, 0);
+    const uint32_t nd = node < last_node ? node : last_node;
+    if (nv) {
+      mac = a.mac[nd];
+      const uint2 lk = a.link[nd];
+      const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
+      const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
+      const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
+      const double near2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+      accept = theta2 * near2 > mac.w;
+      if constexpr (kGroup) {
+        if (!accept) {
+          const double fx0 = fmax(mac.x - blo0, bhi0 - mac.x);
+          const double fx1 = fmax(mac.y - blo1, bhi1 - mac.y);
+          const double fx2 = fmax(mac.z - blo2, bhi2 - mac.z);
+          const double far2 = fx0 * fx0 + fx1 * fx1 + fx2 * fx2;
+          defer = !(theta2 * far2 <= mac.w);
+        }
+      }
+      isleaf = (lk.y >> 31) != 0;
+      lfirst = lk.x;
+      lcnt = lk.y & 0x7fffffffu;
+    }
+    const bool cellf = nv && accept;
+    const bool deff = nv && defer;
+    const bool leaff = nv && !accept && !defer && isleaf;
+    // a node at depth 21 is always a leaf; the guard keeps the frame stack in bounds regardless
+    const bool openf = nv && !accept && !defer && !isleaf && depth + 1 < kLevels;
+    if constexpr (kGroup) {
+      const uint64_t dm = __ballot(deff);
+      if (dm) {
+        const uint32_t nd_new = __popcll(dm);
+        if (n_def + nd_new > kDefer) {
+          ok = false;
+          break;
+        }
+        if (deff) dl[n_def + __popcll(dm & lt)] = nd << 3;
+        n_def += nd_new;
+      }
+    }
+    // children of opened interior nodes -> frame depth + 1
+    const uint64_t om = __ballot(openf);
+    if (om) {
+      const uint32_t pos = __popcll(om & lt);
+      if (openf) L.fr[depth + 1][pos] = (lfirst << 3) | (lcnt - 1u);
+      if (lane == 0) {
+        L.fr_n[depth + 1] = __popcll(om);
+        L.fr_cur[depth + 1] = 0;
+      }
+    }
+    // accepted nodes
+    const uint64_t cm = __ballot(cellf);
+    if (cm) {
+      const uint32_t nc = __popcll(cm);
+      const uint32_t pos = __popcll(cm & lt);
+      if (use_quad) {
+        if (cellf) {
+          const uint32_t slot = pc_cnt + pos;
+          if constexpr (sizeof(Real) == 4) {
+            const float4 q0 = a.q32[2 * nd + 0], q1 = a.q32[2 * nd + 1];
+            L.pc[3 * slot + 0] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q0.x);
+            L.pc[3 * slot + 1] = make_float4(q0.y, q0.z, q0.w, q1.x);
+            L.pc[3 * slot + 2] = make_float4(q1.y, q1.z, 0.f, 0.f);
+          } else {
+            const double* q = &a.q64[8 * nd];
+            L.pc[3 * slot + 0] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q[0]);
+            L.pc[3 * slot + 1] = make_double4(q[1], q[2], q[3], q[4]);
+            L.pc[3 * slot + 2] = make_double4(q[5], q[6], 0.0, 0.0);
+          }
+        }
+        pc_cnt += nc;
+        n_pc += nc;
+        while (pc_cnt >= B) flush_pc();
+      } else {
+        if (cellf) {
+          const double m = sizeof(Real) == 4 ? static_cast<double>(a.q32[2 * nd].x) : a.q64[8 * nd];
+          L.pp[pp_cnt + pos] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, m);
+        }
+        pp_cnt += nc;
+        n_pp += nc;
+        while (pp_cnt >= B) flush_pp();
+      }
+    }
+    // particles of opened leaves; particle ranges of one iteration's
+    // leaves are ascending and disjoint, mapped to slots like the nodes
+    const uint64_t lm = __ballot(leaff);
+    if (lm) {
+      const uint32_t lc = leaff ? lcnt : 0u;
+      const uint32_t lincl = wave_incl_add(lc);
+      const uint32_t lexcl = lincl - lc;
+      const uint32_t ltot = uniform(__builtin_amdgcn_readlane(lincl, 63));
+      const uint32_t lval = lfirst - lexcl;
+      for (uint32_t b = 0; b < ltot; b += kWave) {
+        // the leaf holding slot b: the last leaf lane starting at or before it
+        const uint64_t before = __ballot(leaff && lexcl <= b);
+        const int j0 = 63 - __clzll(static_cast<long long>(before));
+        const uint32_t v0 = uniform(__builtin_amdgcn_readlane(lval, j0));
+        L.own[lane] = v0;
+        __builtin_amdgcn_wave_barrier();
+        if (leaff && lexcl > b && lexcl < b + kWave) L.own[lexcl - b] = lval;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t e = b + lane;
+        uint32_t q = wave_incl_max(L.own[lane]) + e;
+        q = q < last_part ? q : last_part;
+        const uint32_t chunk = ltot - b < kWave ? ltot - b : kWave;
+        if (e < ltot) {
+          const double4 sp = a.posm[q];
+          L.pp[pp_cnt + lane] = make_entry<Real>(sp.x - cx, sp.y - cy, sp.z - cz, sp.w);
+        }
+        pp_cnt += chunk;
+        n_pp += chunk;
+        while (pp_cnt >= B) flush_pp();
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (om) ++depth;
+  }
+  if (!ok) return false;
+  // final partial batches (fewer than B entries), padded to B
+  if (pp_cnt) {
+    if (static_cast<uint32_t>(lane) >= pp_cnt && static_cast<uint32_t>(lane) < B) L.pp[lane] = pad_entry<Real>();
+    pp_cnt = B;
+    flush_pp();
+  }
+  if (pc_cnt) {
+    if (static_cast<uint32_t>(lane) >= pc_cnt && static_cast<uint32_t>(lane) < B) {
+      const V4 z = make_entry<Real>(0.0, 0.0, 0.0, 0.0);
+      L.pc[3 * lane + 0] = pad_entry<Real>();
+      L.pc[3 * lane + 1] = z;
+      L.pc[3 * lane + 2] = z;
+    }
+    pc_cnt = B;
+    flush_pc();
+  }
+  __builtin_amdgcn_wave_barrier();
+  return true;
+}
+
+// FP64 bounding box of particles [b, b + n) (multipole.h:115-128).
+__device__ inline void particle_box(const WalkArgs& a, uint32_t b, uint32_t n, double* box) {
+  const double inf = __builtin_huge_val();
+  double lo0 = inf, lo1 = inf, lo2 = inf, hi0 = -inf, hi1 = -inf, hi2 = -inf;
+  for (uint32_t p = __lane_id(); p < n; p += kWave) {
+    const double4 q = a.posm[b + p];
+    lo0 = fmin(lo0, q.x);
+    lo1 = fmin(lo1, q.y);
+    lo2 = fmin(lo2, q.z);
+    hi0 = fmax(hi0, q.x);
+    hi1 = fmax(hi1, q.y);
+    hi2 = fmax(hi2, q.z);
+  }
+  box[0] = wave_min(lo0);
+  box[1] = wave_min(lo1);
+  box[2] = wave_min(lo2);
+  box[3] = wave_max(hi0);
+  box[4] = wave_max(hi1);
+  box[5] = wave_max(hi2);
+}
+
+// One target (node tnode): the walk over `start` (the group's deferred
+// nodes) or from the root, for each chunk of <= 64 of its particles; then
+// the slices' sums, plus the group's share (gacc, counts g_pp / g_pc) when
+// given, times G, to the outputs and the fused kick/drift.
+template <typename Real, bool kMask>
+__device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode, const uint32_t* start,
+                            uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc) {
   const int lane = __lane_id();
   const uint32_t tb = uniform(a.begin[tnode]);
   const uint32_t T = uniform(a.count[tnode]);
@@ -229,241 +606,37 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
   const bool use_quad = sp.use_quad != 0;
   const double4 tcv = a.ctr[tnode];
   const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
-  const uint32_t last_node = a.n_nodes - 1, last_part = a.n_parts - 1;
-  // FP64 bounding box of the target's particles (multipole.h:115-128)
-  const double inf = __builtin_huge_val();
-  double blo0 = inf, blo1 = inf, blo2 = inf, bhi0 = -inf, bhi1 = -inf, bhi2 = -inf;
-  for (uint32_t p = lane; p < T; p += kWave) {
-    const double4 q = a.posm[tb + p];
-    blo0 = fmin(blo0, q.x);
-    blo1 = fmin(blo1, q.y);
-    blo2 = fmin(blo2, q.z);
-    bhi0 = fmax(bhi0, q.x);
-    bhi1 = fmax(bhi1, q.y);
-    bhi2 = fmax(bhi2, q.z);
-  }
-  blo0 = wave_min(blo0);
-  blo1 = wave_min(blo1);
-  blo2 = wave_min(blo2);
-  bhi0 = wave_max(bhi0);
-  bhi1 = wave_max(bhi1);
-  bhi2 = wave_max(bhi2);
+  double box[6];
+  particle_box(a, tb, T, box);
   const double theta2 = sp.theta2;
-  const Real eps2 = static_cast<Real>(sp.eps2);
-  const uint64_t lt = lanemask_lt();
 
   for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
     const uint32_t tc = T - c0 < kWave ? T - c0 : kWave;
     const int S = kWave / static_cast<int>(tc);
-    const int C = kWave / S;
-    const uint32_t B = static_cast<uint32_t>(S * C);
-    const bool active = lane < static_cast<int>(tc) * S;
-    const int p = lane % static_cast<int>(tc), s = lane / static_cast<int>(tc);
-    const uint32_t pidx = tb + c0 + p;
-    Real tx = 0, ty = 0, tz = 0;
+    TargetEval<Real, kMask> ev;
+    ev.C = kWave / S;
+    ev.B = static_cast<uint32_t>(S * ev.C);
+    ev.active = lane < static_cast<int>(tc) * S;
+    const int p = lane % static_cast<int>(tc);
+    ev.s = lane / static_cast<int>(tc);
+    ev.eps2 = static_cast<Real>(sp.eps2);
     {
-      const double4 q = a.posm[pidx];
-      tx = static_cast<Real>(q.x - tcx);
-      ty = static_cast<Real>(q.y - tcy);
-      tz = static_cast<Real>(q.z - tcz);
+      const double4 q = a.posm[tb + c0 + p];
+      ev.tx = static_cast<Real>(q.x - tcx);
+      ev.ty = static_cast<Real>(q.y - tcy);
+      ev.tz = static_cast<Real>(q.z - tcz);
     }
-    const V4* my_pp = &L.pp[s * C];
-    const V4* my_pc = &L.pc[3 * s * C];
-    Acc<Real> acc;
-    uint32_t pp_cnt = 0, pc_cnt = 0;
-    uint32_t n_pp = 0, n_pc = 0;
-
-    // Evaluate the first B staged entries, then move the rest (< 64: at most
-    // B - 1 + 64 were staged) to the front; callers repeat while >= B remain.
-    auto flush_pp = [&]() {
-      __builtin_amdgcn_wave_barrier();
-      if (active) eval_pp<Real, kMask>(my_pp, C, tx, ty, tz, eps2, acc);
-      __builtin_amdgcn_wave_barrier();
-      const uint32_t rest = pp_cnt - B;
-      V4 v;
-      if (static_cast<uint32_t>(lane) < rest) v = L.pp[B + lane];
-      __builtin_amdgcn_wave_barrier();
-      if (static_cast<uint32_t>(lane) < rest) L.pp[lane] = v;
-      pp_cnt = rest;
-      __builtin_amdgcn_wave_barrier();
-    };
-    auto flush_pc = [&]() {
-      __builtin_amdgcn_wave_barrier();
-      if (active) eval_pc<Real>(my_pc, C, tx, ty, tz, eps2, acc);
-      __builtin_amdgcn_wave_barrier();
-      const uint32_t rest = pc_cnt - B;
-      V4 v0, v1, v2;
-      if (static_cast<uint32_t>(lane) < rest) {
-        v0 = L.pc[3 * (B + lane) + 0];
-        v1 = L.pc[3 * (B + lane) + 1];
-        v2 = L.pc[3 * (B + lane) + 2];
-      }
-      __builtin_amdgcn_wave_barrier();
-      if (static_cast<uint32_t>(lane) < rest) {
-        L.pc[3 * lane + 0] = v0;
-        L.pc[3 * lane + 1] = v1;
-        L.pc[3 * lane + 2] = v2;
-      }
-      pc_cnt = rest;
-      __builtin_amdgcn_wave_barrier();
-    };
-
-    if (lane == 0) {
-      L.fr[0][0] = sys_id << 3;  // the system's root: node sys_id, one node
-      L.fr_n[0] = 1;
-      L.fr_cur[0] = 0;
-    }
-    __builtin_amdgcn_wave_barrier();
-    int depth = 0;
-    while (depth >= 0) {
-      const uint32_t nr = uniform(L.fr_n[depth]);
-      const uint32_t cur = uniform(L.fr_cur[depth]);
-      if (cur >= nr) {
-        --depth;
-        continue;
-      }
-      // Pop whole child ranges, up to 64 nodes. Ranges of a frame are
-      // ascending and disjoint, so first - excl is non-decreasing over the
-      // ranges and a prefix max of it, placed at each range's first slot,
-      // yields every slot's node: node = (first_j - excl_j) + slot.
-      const uint32_t rr = cur + lane;
-      const bool rv = rr < nr;
-      const uint32_t packed = rv ? L.fr[depth][rr] : 0u;
-      const uint32_t rcnt = rv ? (packed & 7u) + 1u : 0u;
-      const uint32_t rfirst = packed >> 3;
-      const uint32_t incl = wave_incl_add(rcnt);
-      const bool taken = rv && incl <= kWave;
-      const int k = __popcll(__ballot(taken));
-      const uint32_t total = uniform(__builtin_amdgcn_readlane(incl, k - 1));
-      const uint32_t excl = incl - rcnt;
-      L.own[lane] = 0u;
-      __builtin_amdgcn_wave_barrier();
-      if (taken) L.own[excl] = rfirst - excl;
-      if (lane == 0) L.fr_cur[depth] = cur + k;
-      __builtin_amdgcn_wave_barrier();
-      const uint32_t node = wave_incl_max(L.own[lane]) + static_cast<uint32_t>(lane);
-      const bool nv = static_cast<uint32_t>(lane) < total;
-      // MAC in FP64, reference operation order
-      bool accept = false, isleaf = false;
-      uint32_t lfirst = 0, lcnt = 0;
-      double4 mac = make_double4(0, 0, 0, 0);
-      const uint32_t nd = node < last_node ? node : last_node;
-      if (nv) {
-        mac = a.mac[nd];
-        const uint2 lk = a.link[nd];
-        const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
-        const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
-        const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
-        const double near2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-        accept = theta2 * near2 > mac.w;
-        isleaf = (lk.y >> 31) != 0;
-        lfirst = lk.x;
-        lcnt = lk.y & 0x7fffffffu;
-      }
-      const bool cellf = nv && accept;
-      const bool leaff = nv && !accept && isleaf;
-      // a node at depth 21 is always a leaf; the guard keeps the frame stack in bounds regardless
-      const bool openf = nv && !accept && !isleaf && depth + 1 < kLevels;
-      // children of opened interior nodes -> frame depth + 1
-      const uint64_t om = __ballot(openf);
-      if (om) {
-        const uint32_t pos = __popcll(om & lt);
-        if (openf) L.fr[depth + 1][pos] = (lfirst << 3) | (lcnt - 1u);
-        if (lane == 0) {
-          L.fr_n[depth + 1] = __popcll(om);
-          L.fr_cur[depth + 1] = 0;
-        }
-      }
-      // accepted nodes
-      const uint64_t cm = __ballot(cellf);
-      if (cm) {
-        const uint32_t nc = __popcll(cm);
-        const uint32_t pos = __popcll(cm & lt);
-        if (use_quad) {
-          if (cellf) {
-            const uint32_t slot = pc_cnt + pos;
-            if constexpr (sizeof(Real) == 4) {
-              const float4 q0 = a.q32[2 * nd + 0], q1 = a.q32[2 * nd + 1];
-              L.pc[3 * slot + 0] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, q0.x);
-              L.pc[3 * slot + 1] = make_float4(q0.y, q0.z, q0.w, q1.x);
-              L.pc[3 * slot + 2] = make_float4(q1.y, q1.z, 0.f, 0.f);
-            } else {
-              const double* q = &a.q64[8 * nd];
-              L.pc[3 * slot + 0] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, q[0]);
-              L.pc[3 * slot + 1] = make_double4(q[1], q[2], q[3], q[4]);
-              L.pc[3 * slot + 2] = make_double4(q[5], q[6], 0.0, 0.0);
-            }
-          }
-          pc_cnt += nc;
-          n_pc += nc;
-          while (pc_cnt >= B) flush_pc();
-        } else {
-          if (cellf) {
-            const double m = sizeof(Real) == 4 ? static_cast<double>(a.q32[2 * nd].x) : a.q64[8 * nd];
-            L.pp[pp_cnt + pos] = make_entry<Real>(mac.x - tcx, mac.y - tcy, mac.z - tcz, m);
-          }
-          pp_cnt += nc;
-          n_pp += nc;
-          while (pp_cnt >= B) flush_pp();
-        }
-      }
-      // particles of opened leaves; particle ranges of one iteration's
-      // leaves are ascending and disjoint, mapped to slots like the nodes
-      const uint64_t lm = __ballot(leaff);
-      if (lm) {
-        const uint32_t lc = leaff ? lcnt : 0u;
-        const uint32_t lincl = wave_incl_add(lc);
-        const uint32_t lexcl = lincl - lc;
-        const uint32_t ltot = uniform(__builtin_amdgcn_readlane(lincl, 63));
-        const uint32_t lval = lfirst - lexcl;
-        for (uint32_t b = 0; b < ltot; b += kWave) {
-          // the leaf holding slot b: the last leaf lane starting at or before it
-          const uint64_t before = __ballot(leaff && lexcl <= b);
-          const int j0 = 63 - __clzll(static_cast<long long>(before));
-          const uint32_t v0 = uniform(__builtin_amdgcn_readlane(lval, j0));
-          L.own[lane] = v0;
-          __builtin_amdgcn_wave_barrier();
-          if (leaff && lexcl > b && lexcl < b + kWave) L.own[lexcl - b] = lval;
-          __builtin_amdgcn_wave_barrier();
-          const uint32_t e = b + lane;
-          uint32_t q = wave_incl_max(L.own[lane]) + e;
-          q = q < last_part ? q : last_part;
-          const uint32_t chunk = ltot - b < kWave ? ltot - b : kWave;
-          if (e < ltot) {
-            const double4 sp = a.posm[q];
-            L.pp[pp_cnt + lane] = make_entry<Real>(sp.x - tcx, sp.y - tcy, sp.z - tcz, sp.w);
-          }
-          pp_cnt += chunk;
-          n_pp += chunk;
-          while (pp_cnt >= B) flush_pp();
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      if (om) ++depth;
-    }
-    // final partial batches (fewer than B entries), padded to B
-    if (pp_cnt) {
-      if (static_cast<uint32_t>(lane) >= pp_cnt && static_cast<uint32_t>(lane) < B) L.pp[lane] = pad_entry<Real>();
-      pp_cnt = B;
-      flush_pp();
-    }
-    if (pc_cnt) {
-      if (static_cast<uint32_t>(lane) >= pc_cnt && static_cast<uint32_t>(lane) < B) {
-        const V4 z = make_entry<Real>(0.0, 0.0, 0.0, 0.0);
-        L.pc[3 * lane + 0] = pad_entry<Real>();
-        L.pc[3 * lane + 1] = z;
-        L.pc[3 * lane + 2] = z;
-      }
-      pc_cnt = B;
-      flush_pc();
-    }
-    __builtin_amdgcn_wave_barrier();
-    // combine slices in FP64 (the P2C buffer is free now)
+    uint32_t n_pp = 0, n_pc = 0, n_def = 0;
+    traverse<Real, kMask, false>(a, L, box, tcx, tcy, tcz, sys_id, start, n_start, use_quad, theta2, ev, n_pp, n_pc,
+                                 nullptr, n_def);
+    n_pp += g_pp;
+    n_pc += g_pc;
+    // combine slices in FP64 (the P2C ring is free now)
     double* red = reinterpret_cast<double*>(L.pc);
-    if (active) {
-      red[3 * lane + 0] = acc.sx();
-      red[3 * lane + 1] = acc.sy();
-      red[3 * lane + 2] = acc.sz();
+    if (ev.active) {
+      red[3 * lane + 0] = ev.acc.sx();
+      red[3 * lane + 1] = ev.acc.sy();
+      red[3 * lane + 2] = ev.acc.sz();
     }
     __builtin_amdgcn_wave_barrier();
     if (lane < static_cast<int>(tc)) {
@@ -474,8 +647,13 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
         ay += red[3 * src + 1];
         az += red[3 * src + 2];
       }
-      const double gx = sys_G * ax, gy = sys_G * ay, gz = sys_G * az;
       const uint32_t k3 = tb + c0 + lane;
+      if (gacc) {
+        ax += gacc[3 * uint64_t(k3) + 0];
+        ay += gacc[3 * uint64_t(k3) + 1];
+        az += gacc[3 * uint64_t(k3) + 2];
+      }
+      const double gx = sys_G * ax, gy = sys_G * ay, gz = sys_G * az;
       if (a.acc) {
         a.acc[3 * uint64_t(k3) + 0] = gx;
         a.acc[3 * uint64_t(k3) + 1] = gy;
@@ -507,8 +685,75 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode)
   }
 }
 
-// Block b runs on XCD b % 8; give each XCD a contiguous run of targets so
-// neighbouring targets share that XCD's L2.
+// One walk group g (topmost node with leaf || count <= max(tsm, 256)). A
+// group that is itself a target walks as one. Otherwise the wave walks the
+// tree once for the group's box - the decisions every target inside shares
+// - evaluating the shared list for all the group's particles and keeping
+// the deferred nodes; then each target of the group walks only those.
+template <typename Real, bool kMask>
+__device__ void walk_group(const WalkArgs& a, WaveLds<Real>& L, uint32_t unit, uint32_t g) {
+  const int lane = __lane_id();
+  const uint32_t gb = uniform(a.begin[g]);
+  const uint32_t G = uniform(a.count[g]);
+  const uint32_t sys_id = uniform(a.sid[gb]);
+  const SysDev& sp = a.sys[sys_id];
+  const bool gleaf = (a.link[g].y >> 31) != 0;
+  if (gleaf || G <= sp.tsm || G > kGroupCap) {
+    walk_target<Real, kMask>(a, L, g, nullptr, 0, nullptr, 0, 0);
+    return;
+  }
+  const double4 gcv = a.ctr[g];
+  double box[6];
+  particle_box(a, gb, G, box);
+  GroupEval<Real, kMask> ev;
+  ev.nc = static_cast<int>((G + kWave - 1) / kWave);
+  ev.B = kWave;
+  ev.eps2 = static_cast<Real>(sp.eps2);
+#pragma unroll
+  for (int k = 0; k < kGroupChunks; ++k) {
+    const uint32_t i = static_cast<uint32_t>(k * kWave + lane);
+    const uint32_t pi = gb + (i < G ? i : G - 1);
+    const double4 q = a.posm[pi];
+    ev.gx[k] = static_cast<Real>(q.x - gcv.x);
+    ev.gy[k] = static_cast<Real>(q.y - gcv.y);
+    ev.gz[k] = static_cast<Real>(q.z - gcv.z);
+  }
+  uint32_t* dl = a.defer + uint64_t(unit) * kDefer;
+  uint32_t g_pp = 0, g_pc = 0, n_def = 0;
+  const bool ok = traverse<Real, kMask, true>(a, L, box, gcv.x, gcv.y, gcv.z, sys_id, nullptr, 0, sp.use_quad != 0,
+                                              sp.theta2, ev, g_pp, g_pc, dl, n_def);
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < kGroupChunks; ++k) {
+      const uint32_t i = static_cast<uint32_t>(k * kWave + lane);
+      if (k < ev.nc && i < G) {
+        a.gacc[3 * uint64_t(gb + i) + 0] = ev.acc[k].sx();
+        a.gacc[3 * uint64_t(gb + i) + 1] = ev.acc[k].sy();
+        a.gacc[3 * uint64_t(gb + i) + 2] = ev.acc[k].sz();
+      }
+    }
+    // the deferred list and the group sums are read back by other lanes
+    __threadfence_block();
+  }
+  // the group's targets, in particle order (marked at their first particle)
+  for (uint32_t c = 0; c < G; c += kWave) {
+    const uint32_t i = c + lane;
+    const uint32_t m = i < G ? a.tmark[gb + i] : 0u;
+    uint64_t bits = __ballot(m != 0);
+    while (bits) {
+      const int j = __ffsll(static_cast<long long>(bits)) - 1;
+      bits &= bits - 1;
+      const uint32_t tnode = uniform(__builtin_amdgcn_readlane(m, j)) - 1u;
+      if (ok)
+        walk_target<Real, kMask>(a, L, tnode, dl, n_def, a.gacc, g_pp, g_pc);
+      else
+        walk_target<Real, kMask>(a, L, tnode, nullptr, 0, nullptr, 0, 0);
+    }
+  }
+}
+
+// Block b runs on XCD b % 8; give each XCD a contiguous run of groups so
+// neighbouring groups share that XCD's L2.
 __device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
   const uint32_t per = grid / 8;
   return (b % 8) * per + b / 8;
@@ -520,8 +765,8 @@ __global__ __launch_bounds__(kWaves * 64) void k_walk(WalkArgs a) {
   const int w = threadIdx.x >> 6;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t t = blk * kWaves + w;
-  if (t >= a.n_targets) return;
-  walk_target<Real, kMask>(a, lds[w], a.targets[t]);
+  if (t >= a.n_groups) return;
+  walk_group<Real, kMask>(a, lds[w], t, a.groups[t]);
 }
 
 // brute_force_accel (brute_force.h:15-28) for targets tidx[0..m) over a
@@ -618,7 +863,7 @@ __global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ s
 // never triggers when eps2 > 0, so one masked launch serves mixed batches.
 template <typename Real, int kWaves>
 hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) {
-  uint32_t blocks = (a.n_targets + kWaves - 1) / kWaves;
+  uint32_t blocks = (a.n_groups + kWaves - 1) / kWaves;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
   if (!any_unsoftened)
     hipLaunchKernelGGL((k_walk<Real, kWaves, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
@@ -630,7 +875,7 @@ hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) 
 }  // namespace
 
 hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s) {
-  if (a.n_targets == 0) return hipSuccess;
+  if (a.n_groups == 0) return hipSuccess;
   return fp64 ? launch_walk_t<double, 2>(a, any_unsoftened, s) : launch_walk_t<float, 2>(a, any_unsoftened, s);
 }
 

@@ -87,6 +87,16 @@ def cpu_baseline(p, fp, args):
     }
 
 
+def ens_shapes(args, n):
+    """(max_leaf, group_size, target_size): the autotuned table's choice for
+    n, unless fixed on the command line (--ens-max-leaf > 0)."""
+    if args.ens_max_leaf > 0:
+        return args.ens_max_leaf, args.ens_group_size, args.ens_target_size
+    from bhgpu import autotune
+
+    return autotune.shapes_for(n)
+
+
 def ensemble_run(ctx, args):
     """The mixed-N ensemble (64 systems, N log-uniform in [4096, 2^20], half
     Plummer / half Hernquist, theta in {0.4, 0.5, 0.7}, eps 0.01): the
@@ -101,7 +111,7 @@ def ensemble_run(ctx, args):
     p, meta = ensemble.make()
     n = len(p)
     log(f"ensemble: {len(meta)} systems, {n} particles, generated in {time.time() - t0:.1f}s")
-    shapes = lambda m: (args.ens_max_leaf, args.ens_group_size, args.ens_target_size)  # noqa: E731
+    shapes = lambda n: ens_shapes(args, n)  # noqa: E731
     dt = 1e-3
     sched = [(dt / 2, dt)] + [(dt, dt)] * (args.ensemble_steps - 1) + [(dt / 2, 0.0)]
     tables = {kd: ensemble.table(meta, shapes, *kd) for kd in set(sched)}
@@ -123,7 +133,7 @@ def ensemble_run(ctx, args):
             else:
                 for k, m in enumerate(meta):
                     o = int(offs[k]) * 64
-                    ml, gs, ts = shapes(m)
+                    ml, gs, ts = shapes(m["n"])
                     ctx.step_device(a + o, b + o, m["n"], fps[k], ml, gs, ts, kick, drift)
             lat.append(time.perf_counter() - s0)
             cur ^= 1
@@ -170,7 +180,9 @@ def ensemble_run(ctx, args):
         "systems": len(meta),
         "particles": int(n),
         "evaluations": len(sched),
-        "shapes": {"max_leaf": args.ens_max_leaf, "group_size": args.ens_group_size,
+        "shapes": ("autotuned per system (bhgpu/tuning.json): " +
+                   ", ".join(sorted({str(shapes(m["n"])) for m in meta}))) if args.ens_max_leaf == 0 else
+                  {"max_leaf": args.ens_max_leaf, "group_size": args.ens_group_size,
                    "target_size": args.ens_target_size},
         "batched": res["batched"],
         "per_system": res["per_system"],
@@ -197,13 +209,11 @@ def serving_run(ctx, args):
     t0 = time.time()
     parts = [ic.plummer(int(n), 5000 + k) for k, n in enumerate(ns)]
     log(f"serving: {R} requests, {int(ns.sum())} particles, generated in {time.time() - t0:.1f}s")
-    reqs = [bl.request(int(n), 0.5, 1e-4, True, args.ens_max_leaf, args.ens_group_size, args.ens_target_size,
-                       int(k), 1e-3) for n, k in zip(ns, steps)]
+    reqs = [bl.request(int(n), 0.5, 1e-4, True, *ens_shapes(args, int(n)), int(k), 1e-3) for n, k in zip(ns, steps)]
     srv = bl.Server(ctx, mem_cap_bytes=int(args.serve_cap_gb * 1e9))
     # warm the context's buffers with the largest request alone
     big = int(np.argmax(ns))
-    srv.submit(parts[big], bl.request(int(ns[big]), 0.5, 1e-4, True, args.ens_max_leaf, args.ens_group_size,
-                                      args.ens_target_size, 1, 1e-3))
+    srv.submit(parts[big], bl.request(int(ns[big]), 0.5, 1e-4, True, *ens_shapes(args, int(ns[big])), 1, 1e-3))
     while srv.pending() != (0, 0):
         srv.step()
     srv.poll()
@@ -217,11 +227,13 @@ def serving_run(ctx, args):
             ticket_of[srv.submit(parts[i], reqs[i])] = i
             i += 1
         if srv.pending() == (0, 0):
+            if i >= R:
+                break
             time.sleep(max(0.0, min(arrivals[i] - (srv.clock() - start), 0.05)))
             continue
         stt = srv.step()
         if stt.stepped_particles:
-            batch_sizes.append(stt.active_systems + stt.completed)
+            batch_sizes.append(stt.stepped_systems)
             batch_parts.append(stt.stepped_particles)
             step_ms.append(stt.step_ms)
             used.append(stt.used_bytes)
@@ -267,7 +279,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the mixed-N ensemble comparison")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10)
-    ap.add_argument("--ens-max-leaf", type=int, default=16)
+    ap.add_argument("--ens-max-leaf", type=int, default=0, help="0: autotuned shapes per system")
     ap.add_argument("--ens-group-size", type=int, default=256)
     ap.add_argument("--ens-target-size", type=int, default=32)
     ap.add_argument("--ens-error-subset", type=int, default=64)

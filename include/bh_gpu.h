@@ -95,6 +95,13 @@ int bh_ctx_create(bh_ctx** out, int device);
 void bh_ctx_destroy(bh_ctx* ctx);
 const char* bh_last_error(const bh_ctx* ctx);
 int bh_set_precision(bh_ctx* ctx, int precision);
+/* Walk schedule (results are the same either way: the interaction lists
+ * are identical, sums may round differently): BH_WALK_GROUPED (default) walks
+ * once per group of up to 256 particles for the decisions its targets
+ * share, then per target; BH_WALK_PER_TARGET walks from the root for
+ * every target. */
+typedef enum bh_walk_mode { BH_WALK_GROUPED = 0, BH_WALK_PER_TARGET = 1 } bh_walk_mode;
+int bh_set_walk_mode(bh_ctx* ctx, int mode);
 /* Limits: n < 2^30 particles, max_leaf <= 1024. */
 uint64_t bh_max_particles(void);
 
@@ -170,6 +177,8 @@ typedef struct bh_server_stats {
   uint32_t pending;
   uint32_t admitted;
   uint32_t completed;
+  uint32_t stepped_systems;   /* systems evaluated by this step */
+  uint32_t pad;
   uint64_t active_particles;
   uint64_t stepped_particles; /* particles evaluated by this step */
   uint64_t used_bytes;        /* footprint of the active systems */

@@ -100,6 +100,29 @@ def test_walk_parity(gpu, orc, n, ml, gs, ts, theta, eps2, quad):
     gpu.set_precision(bl.BH_FP32)
 
 
+@pytest.mark.parametrize("mode", [bl.BH_WALK_GROUPED, bl.BH_WALK_PER_TARGET])
+@pytest.mark.parametrize("theta,kind", [(0.1, "plummer"), (0.5, "plummer"), (0.8, "hernquist"), (0.5, "hernquist")])
+def test_walk_modes_clustered(gpu, orc, mode, theta, kind):
+    # theta 0.1 defers more nodes than a group may hold: its groups fall back
+    # to walks from the root
+    p = getattr(ic, kind)(30000, 21)
+    fp = layout.force_params(theta, 1e-4, 1.0, True)
+    ot = orc.build(p, 8, 256)
+    oa, opp, opc, otot = ot.grouped_pass(fp, 32)
+    gpu.set_walk_mode(mode)
+    try:
+        gpu.build_tree(p, 8, 256)
+        for prec in (bl.BH_FP64, bl.BH_FP32):
+            print(f"walk mode={mode} {kind} theta={theta} prec={prec}", flush=True)
+            gpu.set_precision(prec)
+            ga, gpp, gpc, gtot = gpu.accel(fp, 32)
+            assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc) and np.array_equal(gtot, otot)
+            assert rel_err(ga, oa) < TOL[prec]
+    finally:
+        gpu.set_walk_mode(bl.BH_WALK_GROUPED)
+        gpu.set_precision(bl.BH_FP32)
+
+
 def test_large_targets_span_chunks(gpu, orc):
     # targets of more than 64 particles (target_size 200 and a 150-fold duplicate leaf)
     p = ic.random_box(4000, 12, 10.0)
