@@ -35,9 +35,12 @@ _cache = {}
 
 
 def load_table(path=TABLE):
-    if path not in _cache:
-        _cache[path] = json.load(open(path)) if os.path.exists(path) else None
-    return _cache[path]
+    if not os.path.exists(path):
+        return None
+    key = (path, os.path.getmtime(path))
+    if key not in _cache:
+        _cache[key] = json.load(open(path))
+    return _cache[key]
 
 
 def shapes_for(n, path=TABLE):

@@ -398,10 +398,11 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
       --depth;
       continue;
     }
-    // Pop whole child ranges, up to 64 nodes. Ranges of a frame are
-    // ascending and disjoint, so first - excl is non-decreasing over the
-    // ranges and a prefix max of it, placed at each range's first slot,
-    // yields every slot's node: node = (first_j - excl_j) + slot.
+    // Pop whole child ranges, up to 64 nodes. Range j covers slots
+    // [excl_j, excl_j + count_j); its index j, placed at its first slot,
+    // is non-decreasing over the slots, so a prefix max gives every slot its
+    // range: node = first_j + (slot - excl_j). (The ranges themselves need
+    // not be ascending: children of deferred nodes come in walk order.)
     const uint32_t rr = cur + lane;
     const bool rv = rr < nr;
     uint32_t node, total;
@@ -422,10 +423,12 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
       const uint32_t excl = incl - rcnt;
       L.own[lane] = 0u;
       __builtin_amdgcn_wave_barrier();
-      if (taken) L.own[excl] = rfirst - excl;
+      if (taken) L.own[excl] = static_cast<uint32_t>(lane);
       if (lane == 0) L.fr_cur[depth] = cur + k;
       __builtin_amdgcn_wave_barrier();
-      node = wave_incl_max(L.own[lane]) + static_cast<uint32_t>(lane);
+      const int j = static_cast<int>(wave_incl_max(L.own[lane]));
+      node = static_cast<uint32_t>(__shfl(static_cast<int>(rfirst), j)) +
+             (static_cast<uint32_t>(lane) - static_cast<uint32_t>(__shfl(static_cast<int>(excl), j)));
     }
     const bool nv = static_cast<uint32_t>(lane) < total;
     // MAC in FP64, reference operation order (multipole.h:130-160)
@@ -514,26 +517,26 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
         while (pp_cnt >= B) flush_pp();
       }
     }
-    // particles of opened leaves; particle ranges of one iteration's
-    // leaves are ascending and disjoint, mapped to slots like the nodes
+    // particles of opened leaves, mapped to slots like the nodes: the lane
+    // of the leaf covering each slot by a prefix max of lane indices
     const uint64_t lm = __ballot(leaff);
     if (lm) {
       const uint32_t lc = leaff ? lcnt : 0u;
       const uint32_t lincl = wave_incl_add(lc);
       const uint32_t lexcl = lincl - lc;
       const uint32_t ltot = uniform(__builtin_amdgcn_readlane(lincl, 63));
-      const uint32_t lval = lfirst - lexcl;
       for (uint32_t b = 0; b < ltot; b += kWave) {
         // the leaf holding slot b: the last leaf lane starting at or before it
         const uint64_t before = __ballot(leaff && lexcl <= b);
         const int j0 = 63 - __clzll(static_cast<long long>(before));
-        const uint32_t v0 = uniform(__builtin_amdgcn_readlane(lval, j0));
-        L.own[lane] = v0;
+        L.own[lane] = static_cast<uint32_t>(j0);
         __builtin_amdgcn_wave_barrier();
-        if (leaff && lexcl > b && lexcl < b + kWave) L.own[lexcl - b] = lval;
+        if (leaff && lexcl > b && lexcl < b + kWave) L.own[lexcl - b] = static_cast<uint32_t>(lane);
         __builtin_amdgcn_wave_barrier();
         const uint32_t e = b + lane;
-        uint32_t q = wave_incl_max(L.own[lane]) + e;
+        const int jj = static_cast<int>(wave_incl_max(L.own[lane]));
+        uint32_t q = static_cast<uint32_t>(__shfl(static_cast<int>(lfirst), jj)) +
+                     (e - static_cast<uint32_t>(__shfl(static_cast<int>(lexcl), jj)));
         q = q < last_part ? q : last_part;
         const uint32_t chunk = ltot - b < kWave ? ltot - b : kWave;
         if (e < ltot) {

@@ -24,13 +24,23 @@ def check_alone(gpu, specs):
     allp, table = make_batch(specs)
     print(f"batch of {len(specs)} systems, {len(allp)} particles", flush=True)
     out, t = gpu.step_batch(allp, table)
-    off = 0
+    off, bad = 0, []
     for k, (p, th, e2, q, ml, gs, ts, kdt, ddt) in enumerate(specs):
         n = len(p)
         if n:
             alone, _ = gpu.step(p, layout.force_params(th, e2, 1.0, q), ml, gs, ts, kdt, ddt)
-            assert out[off:off + n].tobytes() == alone.tobytes(), f"system {k} (n={n})"
+            got = out[off:off + n]
+            if got.tobytes() != alone.tobytes():
+                same_order = bool(np.array_equal(got["id"], alone["id"]))
+                msg = f"system {k} (n={n} theta={th} eps2={e2} ml={ml} gs={gs} ts={ts}): same order {same_order}"
+                if same_order:
+                    msg += (f", max |dpos| {np.abs(got['pos'] - alone['pos']).max():.3e}"
+                            f", max |dvel| {np.abs(got['vel'] - alone['vel']).max():.3e}"
+                            f", differing particles {int((got['vel'] != alone['vel']).any(1).sum())}")
+                print(msg, flush=True)
+                bad.append(k)
         off += n
+    assert not bad, f"systems differing from their step alone: {bad}"
     return out, t
 
 
