@@ -103,9 +103,7 @@ def ensemble_run(ctx, args):
     batched path (bh_step_batch, every system in the same launches) against
     the plain path (one bh_step_device call per system), each over the
     leapfrog schedule of `ensemble_steps` steps from the same resident start."""
-    import torch
-
-    from bhgpu import ensemble
+    from bhgpu import devmem, ensemble
 
     t0 = time.time()
     p, meta = ensemble.make()
@@ -117,17 +115,16 @@ def ensemble_run(ctx, args):
     tables = {kd: ensemble.table(meta, shapes, *kd) for kd in set(sched)}
     offs = np.concatenate([[0], np.cumsum([m["n"] for m in meta])])
     fps = [layout.force_params(m["theta"], m["eps2"], 1.0, True) for m in meta]
-    host = torch.from_numpy(p.view(np.uint8).copy())
-    buf = [torch.empty(n * 64, dtype=torch.uint8, device="cuda") for _ in range(2)]
+    buf = [devmem.DeviceBuffer(n * 64) for _ in range(2)]
 
     def run(mode, schedule):
-        buf[0].copy_(host)
-        torch.cuda.synchronize()
+        buf[0].upload(p)
+        devmem.synchronize()
         cur, lat = 0, []
         t_start = time.perf_counter()
         for kick, drift in schedule:
             s0 = time.perf_counter()
-            a, b = buf[cur].data_ptr(), buf[1 - cur].data_ptr()
+            a, b = buf[cur].ptr, buf[1 - cur].ptr
             if mode == "batched":
                 ctx.step_batch_device(a, b, tables[(kick, drift)])
             else:
@@ -138,7 +135,7 @@ def ensemble_run(ctx, args):
             lat.append(time.perf_counter() - s0)
             cur ^= 1
         wall = time.perf_counter() - t_start
-        return wall, lat, buf[cur].cpu().numpy().copy()
+        return wall, lat, buf[cur].download(np.empty(n, layout.PARTICLE))
 
     run("batched", sched[:1])  # grow the context's buffers
     run("per_system", sched[:1])
@@ -146,6 +143,7 @@ def ensemble_run(ctx, args):
     finals = {}
     for mode in ("batched", "per_system"):
         wall, lat, fin = run(mode, sched)
+        fin = fin.view(np.uint8)
         finals[mode] = fin
         res[mode] = {
             "particles_per_s": n * len(sched) / wall,
@@ -158,10 +156,10 @@ def ensemble_run(ctx, args):
     # accuracy of the batched step: a = v_out - v_in from one step with
     # kick_dt = 1, drift_dt = 0, against the FP64 direct sum per system
     kd = (1.0, 0.0)
-    buf[0].copy_(host)
-    torch.cuda.synchronize()
-    ctx.step_batch_device(buf[0].data_ptr(), buf[1].data_ptr(), ensemble.table(meta, shapes, *kd))
-    out = buf[1].cpu().numpy().view(layout.PARTICLE)
+    buf[0].upload(p)
+    devmem.synchronize()
+    ctx.step_batch_device(buf[0].ptr, buf[1].ptr, ensemble.table(meta, shapes, *kd))
+    out = buf[1].download(np.empty(n, layout.PARTICLE))
     rng = np.random.default_rng(777)
     errs, worst = [], 0.0
     for k, m in enumerate(meta):
@@ -209,11 +207,15 @@ def serving_run(ctx, args):
     t0 = time.time()
     parts = [ic.plummer(int(n), 5000 + k) for k, n in enumerate(ns)]
     log(f"serving: {R} requests, {int(ns.sum())} particles, generated in {time.time() - t0:.1f}s")
-    reqs = [bl.request(int(n), 0.5, 1e-4, True, *ens_shapes(args, int(n)), int(k), 1e-3) for n, k in zip(ns, steps)]
+    def req(n, k):
+        ml, gs, ts = ens_shapes(args, n)
+        return bl.request(n, theta=0.5, eps2=1e-4, max_leaf=ml, group_size=gs, target_size=ts, steps=k, dt=1e-3)
+
+    reqs = [req(int(n), int(k)) for n, k in zip(ns, steps)]
     srv = bl.Server(ctx, mem_cap_bytes=int(args.serve_cap_gb * 1e9))
     # warm the context's buffers with the largest request alone
     big = int(np.argmax(ns))
-    srv.submit(parts[big], bl.request(int(ns[big]), 0.5, 1e-4, True, *ens_shapes(args, int(ns[big])), 1, 1e-3))
+    srv.submit(parts[big], req(int(ns[big]), 1))
     while srv.pending() != (0, 0):
         srv.step()
     srv.poll()
@@ -288,18 +290,19 @@ def main():
     ap.add_argument("--serve-cap-gb", type=float, default=200.0)
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only copy MIB MiB device-to-device (torch) three times, to calibrate HBM byte counters")
+                    help="only copy MIB MiB device-to-device three times, to calibrate HBM byte counters")
     args = ap.parse_args()
 
     if args.copy_calibration:
-        import torch
+        from bhgpu import devmem
 
-        a = torch.ones(args.copy_calibration * (1 << 18), dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
+        nbytes = args.copy_calibration << 20
+        a, b = devmem.DeviceBuffer(nbytes), devmem.DeviceBuffer(nbytes)
+        a.upload(np.ones(nbytes // 4, np.float32))
         for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4, "repeats": 3}))
+            b.copy_from(a)
+        devmem.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": nbytes, "bytes_written": nbytes, "repeats": 3}))
         return 0
 
     if args.gpus != 1:
