@@ -32,9 +32,9 @@ def make(count=64, n_min=4096, n_max=1 << 20, eps=0.01):
 
 
 def table(meta, shapes, kick_dt, drift_dt):
-    """bh_system records; shapes(n) -> (max_leaf, group_size, target_size)."""
+    """bh_system records; shapes(meta entry) -> (max_leaf, group_size, target_size)."""
     rows = []
     for m in meta:
-        ml, gs, ts = shapes(m["n"])
+        ml, gs, ts = shapes(m)
         rows.append(system(m["n"], force_params(m["theta"], m["eps2"], 1.0, True), ml, gs, ts, kick_dt, drift_dt))
     return np.concatenate(rows) if rows else np.zeros(0, SYSTEM)

@@ -63,6 +63,23 @@ def accuracy(ctx, p, fp, args):
     return {"rms_rel": rms, "p99_rel": p99, "global_rel": glob, "subset": int(len(idx)), "reference": "FP64 direct sum"}
 
 
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_walk_plummer16m.md")
+
+
+def profiled_traffic(args):
+    """HBM bytes per walk launch at this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated on a 1 GiB copy), or
+    None when the profile does not cover this configuration."""
+    if args.n != 1 << 24 or args.fp64 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("| `k_walk<float"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4:
+                return (float(cells[2]) + float(cells[3])) * 1e6
+    return None
+
+
 def cpu_baseline(p, fp, args):
     sys.path.insert(0, REPO)
     import oracle  # the FP64 restatement, timed as the CPU baseline
@@ -87,14 +104,14 @@ def cpu_baseline(p, fp, args):
     }
 
 
-def ens_shapes(args, n):
+def ens_shapes(args, n, theta):
     """(max_leaf, group_size, target_size): the autotuned table's choice for
-    n, unless fixed on the command line (--ens-max-leaf > 0)."""
+    (n, theta), unless fixed on the command line (--ens-max-leaf > 0)."""
     if args.ens_max_leaf > 0:
         return args.ens_max_leaf, args.ens_group_size, args.ens_target_size
     from bhgpu import autotune
 
-    return autotune.shapes_for(n)
+    return autotune.shapes_for(n, theta)
 
 
 def ensemble_run(ctx, args):
@@ -109,7 +126,7 @@ def ensemble_run(ctx, args):
     p, meta = ensemble.make()
     n = len(p)
     log(f"ensemble: {len(meta)} systems, {n} particles, generated in {time.time() - t0:.1f}s")
-    shapes = lambda n: ens_shapes(args, n)  # noqa: E731
+    shapes = lambda m: ens_shapes(args, m["n"], m["theta"])  # noqa: E731
     dt = 1e-3
     sched = [(dt / 2, dt)] + [(dt, dt)] * (args.ensemble_steps - 1) + [(dt / 2, 0.0)]
     tables = {kd: ensemble.table(meta, shapes, *kd) for kd in set(sched)}
@@ -130,7 +147,7 @@ def ensemble_run(ctx, args):
             else:
                 for k, m in enumerate(meta):
                     o = int(offs[k]) * 64
-                    ml, gs, ts = shapes(m["n"])
+                    ml, gs, ts = shapes(m)
                     ctx.step_device(a + o, b + o, m["n"], fps[k], ml, gs, ts, kick, drift)
             lat.append(time.perf_counter() - s0)
             cur ^= 1
@@ -179,7 +196,7 @@ def ensemble_run(ctx, args):
         "particles": int(n),
         "evaluations": len(sched),
         "shapes": ("autotuned per system (bhgpu/tuning.json): " +
-                   ", ".join(sorted({str(shapes(m["n"])) for m in meta}))) if args.ens_max_leaf == 0 else
+                   ", ".join(sorted({str(shapes(m)) for m in meta}))) if args.ens_max_leaf == 0 else
                   {"max_leaf": args.ens_max_leaf, "group_size": args.ens_group_size,
                    "target_size": args.ens_target_size},
         "batched": res["batched"],
@@ -208,7 +225,7 @@ def serving_run(ctx, args):
     parts = [ic.plummer(int(n), 5000 + k) for k, n in enumerate(ns)]
     log(f"serving: {R} requests, {int(ns.sum())} particles, generated in {time.time() - t0:.1f}s")
     def req(n, k):
-        ml, gs, ts = ens_shapes(args, n)
+        ml, gs, ts = ens_shapes(args, n, 0.5)
         return bl.request(n, theta=0.5, eps2=1e-4, max_leaf=ml, group_size=gs, target_size=ts, steps=k, dt=1e-3)
 
     reqs = [req(int(n), int(k)) for n, k in zip(ns, steps)]
@@ -394,7 +411,9 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": profiled_traffic(args),
+            "traffic_source": "profiles/k_walk_plummer16m.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, "
+                              "factors from a 1 GiB device copy)",
         },
     }
     if not args.no_ensemble:

@@ -26,17 +26,20 @@ def test_ensemble_small_make_and_table():
     assert len(p) == sum(m["n"] for m in meta)
     assert [m["kind"] for m in meta] == ["plummer", "hernquist"] * 3
     assert [m["theta"] for m in meta] == [0.4, 0.5, 0.7] * 2
-    t = ensemble.table(meta, lambda n: (8, 64, 16), 1e-3, 2e-3)
+    t = ensemble.table(meta, lambda m: (8, 64, 16), 1e-3, 2e-3)
     assert list(t["n"]) == [m["n"] for m in meta] and np.all(t["max_leaf"] == 8) and np.all(t["kick_dt"] == 1e-3)
     assert np.allclose(t["fp"]["eps2"], 1e-4)
 
 
 def test_autotune_lookup(tmp_path):
     path = tmp_path / "t.json"
-    assert autotune.shapes_for(1000, str(path)) == autotune.DEFAULT
-    json.dump({"classes": [{"n": 4096, "shapes": [8, 64, 16]}, {"n": 1 << 20, "shapes": [32, 256, 32]}]}, open(path, "w"))
-    assert autotune.shapes_for(5000, str(path)) == (8, 64, 16)
-    assert autotune.shapes_for(500000, str(path)) == (32, 256, 32)
+    assert autotune.shapes_for(1000, 0.5, str(path)) == autotune.DEFAULT
+    json.dump({"classes": [{"n": 4096, "theta": 0.5, "shapes": [8, 64, 16]},
+                           {"n": 4096, "theta": 0.7, "shapes": [16, 64, 16]},
+                           {"n": 1 << 20, "theta": 0.5, "shapes": [32, 256, 32]}]}, open(path, "w"))
+    assert autotune.shapes_for(5000, 0.5, str(path)) == (8, 64, 16)
+    assert autotune.shapes_for(5000, 0.7, str(path)) == (16, 64, 16)
+    assert autotune.shapes_for(500000, 0.4, str(path)) == (32, 256, 32)
 
 
 def test_server_footprint_model():
