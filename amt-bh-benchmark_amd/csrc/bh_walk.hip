@@ -61,6 +61,25 @@ template <>
 struct Vec4T<double> {
   using T = double4;
 };
+template <typename Real>
+struct Vec2T;
+template <>
+struct Vec2T<float> {
+  using T = float2;
+};
+template <>
+struct Vec2T<double> {
+  using T = double2;
+};
+
+// The P2C ring: cell i is {x, y, z, m} a[i], {Qxx, Qyy, Qzz, Qxy} b[i],
+// {Qxz, Qyz} c[i] (40 bytes in FP32).
+template <typename Real>
+struct PcRing {
+  typename Vec4T<Real>::T a[kBuf];
+  typename Vec4T<Real>::T b[kBuf];
+  typename Vec2T<Real>::T c[kBuf];
+};
 
 template <typename Real>
 struct WaveLds {
@@ -70,7 +89,7 @@ struct WaveLds {
   uint32_t fr_cur[kLevels];
   uint32_t own[kWave];          // lane -> owning range, via prefix max
   V4 pp[kBuf];                  // x, y, z (relative to the target centre), m
-  V4 pc[kBuf * 3];              // {x, y, z, m}, {Qxx, Qyy, Qzz, Qxy}, {Qxz, Qyz, 0, 0}
+  PcRing<Real> pc;
 };
 
 __device__ inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -182,15 +201,16 @@ __device__ inline void eval_pp(const typename Vec4T<Real>::T* e, int C, Real tx,
 // Monopole + traceless quadrupole over this lane's C consecutive cells
 // (multipole.h:171-200): a = M d / r^3 - Q.d / r^5 + 2.5 (d.Q.d) d / r^7.
 template <typename Real>
-__device__ inline void eval_pc(const typename Vec4T<Real>::T* e, int C, Real tx, Real ty, Real tz, Real eps2,
+__device__ inline void eval_pc(const PcRing<Real>& r, int j0, int C, Real tx, Real ty, Real tz, Real eps2,
                                Acc<Real>& acc) {
   using V4 = typename Vec4T<Real>::T;
+  using V2 = typename Vec2T<Real>::T;
   Real bx = 0, by = 0, bz = 0;
 #pragma unroll 2
   for (int k = 0; k < C; ++k) {
-    const V4 e0 = e[3 * k + 0];
-    const V4 e1 = e[3 * k + 1];
-    const V4 e2 = e[3 * k + 2];
+    const V4 e0 = r.a[j0 + k];
+    const V4 e1 = r.b[j0 + k];
+    const V2 e2 = r.c[j0 + k];
     const Real dx = e0.x - tx, dy = e0.y - ty, dz = e0.z - tz;
     const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
     const Real inv = rsq(r2);
@@ -249,16 +269,17 @@ __device__ inline void eval_pp_group(const typename Vec4T<Real>::T* e, const Rea
 }
 
 template <typename Real, int NC>
-__device__ inline void eval_pc_group(const typename Vec4T<Real>::T* e, const Real* gx, const Real* gy, const Real* gz,
+__device__ inline void eval_pc_group(const PcRing<Real>& r, const Real* gx, const Real* gy, const Real* gz,
                                      Real eps2, Acc<Real>* acc) {
   using V4 = typename Vec4T<Real>::T;
+  using V2 = typename Vec2T<Real>::T;
   Real bx[NC], by[NC], bz[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
   for (int j = 0; j < kWave; ++j) {
-    const V4 e0 = e[3 * j + 0];
-    const V4 e1 = e[3 * j + 1];
-    const V4 e2 = e[3 * j + 2];
+    const V4 e0 = r.a[j];
+    const V4 e1 = r.b[j];
+    const V2 e2 = r.c[j];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
       const Real dx = e0.x - gx[k], dy = e0.y - gy[k], dz = e0.z - gz[k];
@@ -295,8 +316,8 @@ struct TargetEval {
   __device__ inline void pp(const V4* ring) {
     if (active) eval_pp<Real, kMask>(&ring[s * C], C, tx, ty, tz, eps2, acc);
   }
-  __device__ inline void pc(const V4* ring) {
-    if (active) eval_pc<Real>(&ring[3 * s * C], C, tx, ty, tz, eps2, acc);
+  __device__ inline void pc(const PcRing<Real>& ring) {
+    if (active) eval_pc<Real>(ring, s * C, C, tx, ty, tz, eps2, acc);
   }
 };
 
@@ -316,7 +337,7 @@ struct GroupEval {
       default: eval_pp_group<Real, kMask, 4>(ring, gx, gy, gz, eps2, acc); break;
     }
   }
-  __device__ inline void pc(const V4* ring) {
+  __device__ inline void pc(const PcRing<Real>& ring) {
     switch (nc) {
       case 1: eval_pc_group<Real, 1>(ring, gx, gy, gz, eps2, acc); break;
       case 2: eval_pc_group<Real, 2>(ring, gx, gy, gz, eps2, acc); break;
@@ -368,17 +389,18 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
     ev.pc(L.pc);
     __builtin_amdgcn_wave_barrier();
     const uint32_t rest = pc_cnt - B;
-    V4 v0, v1, v2;
+    V4 v0, v1;
+    typename Vec2T<Real>::T v2;
     if (static_cast<uint32_t>(lane) < rest) {
-      v0 = L.pc[3 * (B + lane) + 0];
-      v1 = L.pc[3 * (B + lane) + 1];
-      v2 = L.pc[3 * (B + lane) + 2];
+      v0 = L.pc.a[B + lane];
+      v1 = L.pc.b[B + lane];
+      v2 = L.pc.c[B + lane];
     }
     __builtin_amdgcn_wave_barrier();
     if (static_cast<uint32_t>(lane) < rest) {
-      L.pc[3 * lane + 0] = v0;
-      L.pc[3 * lane + 1] = v1;
-      L.pc[3 * lane + 2] = v2;
+      L.pc.a[lane] = v0;
+      L.pc.b[lane] = v1;
+      L.pc.c[lane] = v2;
     }
     pc_cnt = rest;
     __builtin_amdgcn_wave_barrier();
@@ -494,14 +516,14 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
           const uint32_t slot = pc_cnt + pos;
           if constexpr (sizeof(Real) == 4) {
             const float4 q0 = a.q32[2 * nd + 0], q1 = a.q32[2 * nd + 1];
-            L.pc[3 * slot + 0] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q0.x);
-            L.pc[3 * slot + 1] = make_float4(q0.y, q0.z, q0.w, q1.x);
-            L.pc[3 * slot + 2] = make_float4(q1.y, q1.z, 0.f, 0.f);
+            L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q0.x);
+            L.pc.b[slot] = make_float4(q0.y, q0.z, q0.w, q1.x);
+            L.pc.c[slot] = make_float2(q1.y, q1.z);
           } else {
             const double* q = &a.q64[8 * nd];
-            L.pc[3 * slot + 0] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q[0]);
-            L.pc[3 * slot + 1] = make_double4(q[1], q[2], q[3], q[4]);
-            L.pc[3 * slot + 2] = make_double4(q[5], q[6], 0.0, 0.0);
+            L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q[0]);
+            L.pc.b[slot] = make_double4(q[1], q[2], q[3], q[4]);
+            L.pc.c[slot] = make_double2(q[5], q[6]);
           }
         }
         pc_cnt += nc;
@@ -560,10 +582,10 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
   }
   if (pc_cnt) {
     if (static_cast<uint32_t>(lane) >= pc_cnt && static_cast<uint32_t>(lane) < B) {
-      const V4 z = make_entry<Real>(0.0, 0.0, 0.0, 0.0);
-      L.pc[3 * lane + 0] = pad_entry<Real>();
-      L.pc[3 * lane + 1] = z;
-      L.pc[3 * lane + 2] = z;
+      L.pc.a[lane] = pad_entry<Real>();
+      L.pc.b[lane] = make_entry<Real>(0.0, 0.0, 0.0, 0.0);
+      L.pc.c[lane].x = Real(0);
+      L.pc.c[lane].y = Real(0);
     }
     pc_cnt = B;
     flush_pc();
@@ -635,7 +657,7 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode,
     n_pp += g_pp;
     n_pc += g_pc;
     // combine slices in FP64 (the P2C ring is free now)
-    double* red = reinterpret_cast<double*>(L.pc);
+    double* red = reinterpret_cast<double*>(L.pc.a);  // 3 x 64 doubles
     if (ev.active) {
       red[3 * lane + 0] = ev.acc.sx();
       red[3 * lane + 1] = ev.acc.sy();
