@@ -81,7 +81,7 @@ struct Ctx {
   size_t off_chunks = 0, off_chunk_first = 0, off_sid_hist = 0;
   // last built tree
   bool has_tree = false;
-  uint32_t n = 0, n_nodes = 0, n_leaves = 0;
+  uint32_t n = 0, n_nodes = 0, n_leaves = 0, max_depth = 0;
   uint32_t max_leaf = 0, group_size = 0;
   bh_particle* sorted = nullptr;    // device: Morton-ordered particles of the last tree
   uint64_t* keys_sorted = nullptr;  // device: sorted keys of the last tree
@@ -339,7 +339,11 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n) {
     return BH_ERR_NONFINITE;
   }
   uint64_t nodes = 0;
-  for (int d = 0; d < kLevels; ++d) nodes += c->host_misc[8 + d];
+  c->max_depth = 0;
+  for (int d = 0; d < kLevels; ++d) {
+    nodes += c->host_misc[8 + d];
+    if (c->host_misc[8 + d]) c->max_depth = static_cast<uint32_t>(d);
+  }
   const uint32_t leaves = c->host_misc[8 + kLevels];
   if (nodes >= (1ull << 29)) {
     set_err(c, "tree of %llu nodes exceeds the supported 2^29", (unsigned long long)nodes);
@@ -420,6 +424,7 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   a.defer = c->defer.p;
   a.n_groups = n_groups;
   a.n_nodes = c->n_nodes;
+  a.max_depth = c->max_depth;
   a.n_parts = c->n;
   a.parts = parts;
   a.acc = d_acc;

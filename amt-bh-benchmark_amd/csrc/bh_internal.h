@@ -42,7 +42,7 @@ static_assert(sizeof(SysDev) == 64, "SysDev layout");
 constexpr uint32_t kChunk = 65536;   // particles per bounds block
 constexpr uint32_t kMaxSystems = 65535;
 constexpr uint64_t kMaxN = (1ull << 30) - 1;  // particles per step (all systems)
-constexpr uint32_t kGroupCap = 256;  // walk groups: topmost nodes with leaf || count <= max(tsm, kGroupCap)
+constexpr uint32_t kGroupCap = 128;  // walk groups: topmost nodes with leaf || count <= max(tsm, kGroupCap)
 constexpr uint32_t kDefer = 512;     // deferred nodes a group may hand its targets
 
 // The device a context was created on.
@@ -88,6 +88,7 @@ struct WalkArgs {
   double* gacc;            // [3n] scratch: a group's shared-list sums per particle
   uint32_t* defer;         // [n_groups * kDefer] scratch: deferred nodes per group
   uint32_t n_groups;
+  uint32_t max_depth;      // deepest level of the tree (sizes the walk's stack)
   uint32_t n_nodes;        // index clamps: a node or particle index never
   uint32_t n_parts;        // leaves its array, whatever the tree holds
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)

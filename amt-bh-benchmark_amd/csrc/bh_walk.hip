@@ -48,7 +48,8 @@ namespace bhk {
 
 namespace {
 
-constexpr int kBuf = 128;  // staging capacity (entries); evaluated in batches of <= 64
+constexpr int kBatch = 48;  // entries evaluated per batch (at most)
+constexpr int kBuf = 112;   // staging capacity: kBatch - 1 left over + 64 staged at once
 constexpr int kGroupChunks = kGroupCap / kWave;  // a group's particles per lane
 
 template <typename Real>
@@ -81,12 +82,15 @@ struct PcRing {
   typename Vec2T<Real>::T c[kBuf];
 };
 
-template <typename Real>
+// kFrames frames: a walk from a node at depth d reaches depth
+// d + kFrames - 1; the launch picks kFrames > the tree's deepest level.
+template <typename Real, int F>
 struct WaveLds {
+  static constexpr int kFrames = F;
   using V4 = typename Vec4T<Real>::T;
-  uint32_t fr[kLevels][kWave];  // child ranges: (first << 3) | (count - 1)
-  uint32_t fr_n[kLevels];
-  uint32_t fr_cur[kLevels];
+  uint32_t fr[F][kWave];  // child ranges: (first << 3) | (count - 1)
+  uint32_t fr_n[F];
+  uint32_t fr_cur[F];
   uint32_t own[kWave];          // lane -> owning range, via prefix max
   V4 pp[kBuf];                  // x, y, z (relative to the target centre), m
   PcRing<Real> pc;
@@ -250,7 +254,7 @@ __device__ inline void eval_pp_group(const typename Vec4T<Real>::T* e, const Rea
 #pragma unroll
   for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
 #pragma unroll 2
-  for (int j = 0; j < kWave; ++j) {
+  for (int j = 0; j < kBatch; ++j) {
     const V4 q = e[j];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -276,7 +280,7 @@ __device__ inline void eval_pc_group(const PcRing<Real>& r, const Real* gx, cons
   Real bx[NC], by[NC], bz[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
-  for (int j = 0; j < kWave; ++j) {
+  for (int j = 0; j < kBatch; ++j) {
     const V4 e0 = r.a[j];
     const V4 e1 = r.b[j];
     const V2 e2 = r.c[j];
@@ -357,8 +361,8 @@ struct GroupEval {
 // that is neither accepted nor rejected for every box inside
 // (theta^2 * far2 > edge^2) is deferred: appended to dl, for its targets.
 // Returns false when more than kDefer nodes would be deferred.
-template <typename Real, bool kMask, bool kGroup, typename Eval>
-__device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box, double cx, double cy, double cz,
+template <typename Real, bool kMask, bool kGroup, typename Eval, typename Lds>
+__device__ bool traverse(const WalkArgs& a, Lds& L, const double* box, double cx, double cy, double cz,
                          uint32_t sys_id, const uint32_t* start, uint32_t n_start, bool use_quad, double theta2,
                          Eval& ev, uint32_t& n_pp, uint32_t& n_pc, uint32_t* dl, uint32_t& n_def) {
   using V4 = typename Vec4T<Real>::T;
@@ -482,8 +486,8 @@ __device__ bool traverse(const WalkArgs& a, WaveLds<Real>& L, const double* box,
     const bool cellf = nv && accept;
     const bool deff = nv && defer;
     const bool leaff = nv && !accept && !defer && isleaf;
-    // a node at depth 21 is always a leaf; the guard keeps the frame stack in bounds regardless
-    const bool openf = nv && !accept && !defer && !isleaf && depth + 1 < kLevels;
+    // the launch sizes the stack for the tree's depth; the guard keeps it in bounds regardless
+    const bool openf = nv && !accept && !defer && !isleaf && depth + 1 < Lds::kFrames;
     if constexpr (kGroup) {
       const uint64_t dm = __ballot(deff);
       if (dm) {
@@ -619,8 +623,8 @@ __device__ inline void particle_box(const WalkArgs& a, uint32_t b, uint32_t n, d
 // nodes) or from the root, for each chunk of <= 64 of its particles; then
 // the slices' sums, plus the group's share (gacc, counts g_pp / g_pc) when
 // given, times G, to the outputs and the fused kick/drift.
-template <typename Real, bool kMask>
-__device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode, const uint32_t* start,
+template <typename Real, bool kMask, typename Lds>
+__device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t tnode, const uint32_t* start,
                             uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc) {
   const int lane = __lane_id();
   const uint32_t tb = uniform(a.begin[tnode]);
@@ -637,9 +641,9 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode,
 
   for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
     const uint32_t tc = T - c0 < kWave ? T - c0 : kWave;
-    const int S = kWave / static_cast<int>(tc);
+    const int S = kWave / static_cast<int>(tc) < kBatch ? kWave / static_cast<int>(tc) : kBatch;
     TargetEval<Real, kMask> ev;
-    ev.C = kWave / S;
+    ev.C = kBatch / S;
     ev.B = static_cast<uint32_t>(S * ev.C);
     ev.active = lane < static_cast<int>(tc) * S;
     const int p = lane % static_cast<int>(tc);
@@ -715,8 +719,8 @@ __device__ void walk_target(const WalkArgs& a, WaveLds<Real>& L, uint32_t tnode,
 // tree once for the group's box - the decisions every target inside shares
 // - evaluating the shared list for all the group's particles and keeping
 // the deferred nodes; then each target of the group walks only those.
-template <typename Real, bool kMask>
-__device__ void walk_group(const WalkArgs& a, WaveLds<Real>& L, uint32_t unit, uint32_t g) {
+template <typename Real, bool kMask, typename Lds>
+__device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g) {
   const int lane = __lane_id();
   const uint32_t gb = uniform(a.begin[g]);
   const uint32_t G = uniform(a.count[g]);
@@ -732,7 +736,7 @@ __device__ void walk_group(const WalkArgs& a, WaveLds<Real>& L, uint32_t unit, u
   particle_box(a, gb, G, box);
   GroupEval<Real, kMask> ev;
   ev.nc = static_cast<int>((G + kWave - 1) / kWave);
-  ev.B = kWave;
+  ev.B = kBatch;
   ev.eps2 = static_cast<Real>(sp.eps2);
 #pragma unroll
   for (int k = 0; k < kGroupChunks; ++k) {
@@ -784,9 +788,9 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
   return (b % 8) * per + b / 8;
 }
 
-template <typename Real, int kWaves, bool kMask>
-__global__ __launch_bounds__(kWaves * 64) void k_walk(WalkArgs a) {
-  __shared__ WaveLds<Real> lds[kWaves];
+template <typename Real, int kWaves, bool kMask, int F>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(WalkArgs a) {
+  __shared__ WaveLds<Real, F> lds[kWaves];
   const int w = threadIdx.x >> 6;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t t = blk * kWaves + w;
@@ -890,10 +894,19 @@ template <typename Real, int kWaves>
 hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) {
   uint32_t blocks = (a.n_groups + kWaves - 1) / kWaves;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
-  if (!any_unsoftened)
-    hipLaunchKernelGGL((k_walk<Real, kWaves, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_walk<Real, kWaves, true>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+  // the stack needs one frame per level below a walk's start
+  const bool shallow = a.max_depth + 1 <= 12;
+  if (!any_unsoftened) {
+    if (shallow)
+      hipLaunchKernelGGL((k_walk<Real, kWaves, false, 12>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_walk<Real, kWaves, false, kLevels>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+  } else {
+    if (shallow)
+      hipLaunchKernelGGL((k_walk<Real, kWaves, true, 12>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_walk<Real, kWaves, true, kLevels>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+  }
   return hipGetLastError();
 }
 
