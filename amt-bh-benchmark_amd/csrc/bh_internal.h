@@ -44,6 +44,7 @@ constexpr uint32_t kMaxSystems = 65535;
 constexpr uint64_t kMaxN = (1ull << 30) - 1;  // particles per step (all systems)
 constexpr uint32_t kGroupCap = 128;  // walk groups: topmost nodes with leaf || count <= max(tsm, kGroupCap)
 constexpr uint32_t kDefer = 512;     // deferred nodes a group may hand its targets
+constexpr uint32_t kShallowFrames = 12;  // walk stack of trees with at most this many levels
 
 // The device a context was created on.
 int ctx_device(const bh_ctx* ctx);

@@ -894,16 +894,18 @@ template <typename Real, int kWaves>
 hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) {
   uint32_t blocks = (a.n_groups + kWaves - 1) / kWaves;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
-  // the stack needs one frame per level below a walk's start
-  const bool shallow = a.max_depth + 1 <= 12;
+  // The stack needs one frame per level below a walk's start: trees of at
+  // most kShallowFrames levels (all systems of a batch) take the
+  // small-stack kernel, which fits 4 waves per SIMD.
+  const bool shallow = a.max_depth + 1 <= kShallowFrames;
   if (!any_unsoftened) {
     if (shallow)
-      hipLaunchKernelGGL((k_walk<Real, kWaves, false, 12>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+      hipLaunchKernelGGL((k_walk<Real, kWaves, false, kShallowFrames>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
     else
       hipLaunchKernelGGL((k_walk<Real, kWaves, false, kLevels>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
   } else {
     if (shallow)
-      hipLaunchKernelGGL((k_walk<Real, kWaves, true, 12>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+      hipLaunchKernelGGL((k_walk<Real, kWaves, true, kShallowFrames>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
     else
       hipLaunchKernelGGL((k_walk<Real, kWaves, true, kLevels>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
   }

@@ -57,6 +57,10 @@ def test_batch_equals_alone_mixed(gpu, prec):
     for k, (n, th, e2, q, ml, gs, ts) in enumerate(MIXED):
         p = ic.plummer(n, 100 + k) if k % 2 == 0 else ic.hernquist(n, 100 + k)
         specs.append((p, th, e2, q, ml, gs, ts, 5e-4 * (1 + k % 3), 1e-3 * (k % 2)))
+    # a deep tree (a 120-fold duplicate: a chain down to depth 21) among shallow ones
+    deep = ic.random_box(400, 77, 2.0, vel_spread=0.1)
+    deep["pos"][:120] = deep["pos"][5]
+    specs.append((deep, 0.5, 1e-4, True, 8, 64, 32, 5e-4, 1e-3))
     gpu.set_precision(prec)
     try:
         out, t = check_alone(gpu, specs)
