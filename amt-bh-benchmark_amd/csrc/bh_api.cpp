@@ -56,6 +56,7 @@ struct Ctx {
   DevBuf<uint32_t> mark, targets;   // walk targets: marks per particle, list
   DevBuf<uint32_t> gmark, groups;   // walk groups: marks per particle, list
   DevBuf<uint32_t> defer;           // deferred nodes per group
+  DevBuf<uint32_t> deep_frames;     // walk stack frames of deep trees, per group
   DevBuf<double> gacc;              // group sums per particle
   // tree (level-order BFS)
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
@@ -425,6 +426,11 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   a.n_groups = n_groups;
   a.n_nodes = c->n_nodes;
   a.max_depth = c->max_depth;
+  a.deep_frames = nullptr;
+  if (c->max_depth + 1 > kShallowFrames) {
+    TRY(ensure(c, c->deep_frames, uint64_t(n_groups) * (kLevels - kShallowFrames) * kWave, "walk stack"));
+    a.deep_frames = c->deep_frames.p;
+  }
   a.n_parts = c->n;
   a.parts = parts;
   a.acc = d_acc;
@@ -601,6 +607,7 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->gmark);
   release(c->groups);
   release(c->defer);
+  release(c->deep_frames);
   release(c->gacc);
   release(c->begin);
   release(c->count);

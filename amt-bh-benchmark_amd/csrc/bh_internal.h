@@ -90,6 +90,7 @@ struct WalkArgs {
   uint32_t* defer;         // [n_groups * kDefer] scratch: deferred nodes per group
   uint32_t n_groups;
   uint32_t max_depth;      // deepest level of the tree (sizes the walk's stack)
+  uint32_t* deep_frames;   // [n_groups * (kLevels - kShallowFrames) * 64] stack frames below the LDS ones
   uint32_t n_nodes;        // index clamps: a node or particle index never
   uint32_t n_parts;        // leaves its array, whatever the tree holds
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)

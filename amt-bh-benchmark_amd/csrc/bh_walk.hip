@@ -82,15 +82,17 @@ struct PcRing {
   typename Vec2T<Real>::T c[kBuf];
 };
 
-// kFrames frames: a walk from a node at depth d reaches depth
-// d + kFrames - 1; the launch picks kFrames > the tree's deepest level.
-template <typename Real, int F>
+// The walk's stack: frame d holds the child ranges d levels below the
+// walk's start. Frames [0, F) live in LDS; with kDeep, frames [F, kLevels)
+// live in a per-group global scratch area (trees deeper than F levels).
+template <typename Real, int F, bool kDeep>
 struct WaveLds {
   static constexpr int kFrames = F;
+  static constexpr bool kDeepFrames = kDeep;
   using V4 = typename Vec4T<Real>::T;
   uint32_t fr[F][kWave];  // child ranges: (first << 3) | (count - 1)
-  uint32_t fr_n[F];
-  uint32_t fr_cur[F];
+  uint32_t fr_n[kLevels];
+  uint32_t fr_cur[kLevels];
   uint32_t own[kWave];          // lane -> owning range, via prefix max
   V4 pp[kBuf];                  // x, y, z (relative to the target centre), m
   PcRing<Real> pc;
@@ -362,9 +364,11 @@ struct GroupEval {
 // (theta^2 * far2 > edge^2) is deferred: appended to dl, for its targets.
 // Returns false when more than kDefer nodes would be deferred.
 template <typename Real, bool kMask, bool kGroup, typename Eval, typename Lds>
-__device__ bool traverse(const WalkArgs& a, Lds& L, const double* box, double cx, double cy, double cz,
+__device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double* box, double cx, double cy, double cz,
                          uint32_t sys_id, const uint32_t* start, uint32_t n_start, bool use_quad, double theta2,
                          Eval& ev, uint32_t& n_pp, uint32_t& n_pc, uint32_t* dl, uint32_t& n_def) {
+  constexpr int F = Lds::kFrames;
+  constexpr int kStack = Lds::kDeepFrames ? kLevels : F;
   using V4 = typename Vec4T<Real>::T;
   const int lane = __lane_id();
   const uint64_t lt = lanemask_lt();
@@ -439,7 +443,12 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, const double* box, double cx
       if (lane == 0) L.fr_cur[0] = cur + total;
       __builtin_amdgcn_wave_barrier();
     } else {
-      const uint32_t packed = rv ? L.fr[depth][rr] : 0u;
+      uint32_t packed = 0u;
+      if constexpr (Lds::kDeepFrames) {
+        if (rv) packed = depth < F ? L.fr[depth][rr] : gfr[(depth - F) * kWave + rr];
+      } else {
+        if (rv) packed = L.fr[depth][rr];
+      }
       const uint32_t rcnt = rv ? (packed & 7u) + 1u : 0u;
       const uint32_t rfirst = packed >> 3;
       const uint32_t incl = wave_incl_add(rcnt);
@@ -487,7 +496,7 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, const double* box, double cx
     const bool deff = nv && defer;
     const bool leaff = nv && !accept && !defer && isleaf;
     // the launch sizes the stack for the tree's depth; the guard keeps it in bounds regardless
-    const bool openf = nv && !accept && !defer && !isleaf && depth + 1 < Lds::kFrames;
+    const bool openf = nv && !accept && !defer && !isleaf && depth + 1 < kStack;
     if constexpr (kGroup) {
       const uint64_t dm = __ballot(deff);
       if (dm) {
@@ -504,7 +513,16 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, const double* box, double cx
     const uint64_t om = __ballot(openf);
     if (om) {
       const uint32_t pos = __popcll(om & lt);
-      if (openf) L.fr[depth + 1][pos] = (lfirst << 3) | (lcnt - 1u);
+      if constexpr (Lds::kDeepFrames) {
+        if (depth + 1 < F) {
+          if (openf) L.fr[depth + 1][pos] = (lfirst << 3) | (lcnt - 1u);
+        } else {
+          if (openf) gfr[(depth + 1 - F) * kWave + pos] = (lfirst << 3) | (lcnt - 1u);
+          __threadfence_block();  // read back by other lanes
+        }
+      } else {
+        if (openf) L.fr[depth + 1][pos] = (lfirst << 3) | (lcnt - 1u);
+      }
       if (lane == 0) {
         L.fr_n[depth + 1] = __popcll(om);
         L.fr_cur[depth + 1] = 0;
@@ -624,7 +642,7 @@ __device__ inline void particle_box(const WalkArgs& a, uint32_t b, uint32_t n, d
 // the slices' sums, plus the group's share (gacc, counts g_pp / g_pc) when
 // given, times G, to the outputs and the fused kick/drift.
 template <typename Real, bool kMask, typename Lds>
-__device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t tnode, const uint32_t* start,
+__device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t tnode, const uint32_t* start,
                             uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc) {
   const int lane = __lane_id();
   const uint32_t tb = uniform(a.begin[tnode]);
@@ -656,8 +674,8 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t tnode, const uin
       ev.tz = static_cast<Real>(q.z - tcz);
     }
     uint32_t n_pp = 0, n_pc = 0, n_def = 0;
-    traverse<Real, kMask, false>(a, L, box, tcx, tcy, tcz, sys_id, start, n_start, use_quad, theta2, ev, n_pp, n_pc,
-                                 nullptr, n_def);
+    traverse<Real, kMask, false>(a, L, gfr, box, tcx, tcy, tcz, sys_id, start, n_start, use_quad, theta2, ev, n_pp,
+                                 n_pc, nullptr, n_def);
     n_pp += g_pp;
     n_pc += g_pc;
     // combine slices in FP64 (the P2C ring is free now)
@@ -727,8 +745,9 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   const uint32_t sys_id = uniform(a.sid[gb]);
   const SysDev& sp = a.sys[sys_id];
   const bool gleaf = (a.link[g].y >> 31) != 0;
+  uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(unit) * (kLevels - Lds::kFrames) * kWave : nullptr;
   if (gleaf || G <= sp.tsm || G > kGroupCap) {
-    walk_target<Real, kMask>(a, L, g, nullptr, 0, nullptr, 0, 0);
+    walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0);
     return;
   }
   const double4 gcv = a.ctr[g];
@@ -749,8 +768,8 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   }
   uint32_t* dl = a.defer + uint64_t(unit) * kDefer;
   uint32_t g_pp = 0, g_pc = 0, n_def = 0;
-  const bool ok = traverse<Real, kMask, true>(a, L, box, gcv.x, gcv.y, gcv.z, sys_id, nullptr, 0, sp.use_quad != 0,
-                                              sp.theta2, ev, g_pp, g_pc, dl, n_def);
+  const bool ok = traverse<Real, kMask, true>(a, L, gfr, box, gcv.x, gcv.y, gcv.z, sys_id, nullptr, 0,
+                                              sp.use_quad != 0, sp.theta2, ev, g_pp, g_pc, dl, n_def);
   if (ok) {
 #pragma unroll
     for (int k = 0; k < kGroupChunks; ++k) {
@@ -774,9 +793,9 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
       bits &= bits - 1;
       const uint32_t tnode = uniform(__builtin_amdgcn_readlane(m, j)) - 1u;
       if (ok)
-        walk_target<Real, kMask>(a, L, tnode, dl, n_def, a.gacc, g_pp, g_pc);
+        walk_target<Real, kMask>(a, L, gfr, tnode, dl, n_def, a.gacc, g_pp, g_pc);
       else
-        walk_target<Real, kMask>(a, L, tnode, nullptr, 0, nullptr, 0, 0);
+        walk_target<Real, kMask>(a, L, gfr, tnode, nullptr, 0, nullptr, 0, 0);
     }
   }
 }
@@ -788,9 +807,9 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
   return (b % 8) * per + b / 8;
 }
 
-template <typename Real, int kWaves, bool kMask, int F>
+template <typename Real, int kWaves, bool kMask, bool kDeep>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(WalkArgs a) {
-  __shared__ WaveLds<Real, F> lds[kWaves];
+  __shared__ WaveLds<Real, kShallowFrames, kDeep> lds[kWaves];
   const int w = threadIdx.x >> 6;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t t = blk * kWaves + w;
@@ -894,20 +913,20 @@ template <typename Real, int kWaves>
 hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) {
   uint32_t blocks = (a.n_groups + kWaves - 1) / kWaves;
   blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
-  // The stack needs one frame per level below a walk's start: trees of at
-  // most kShallowFrames levels (all systems of a batch) take the
-  // small-stack kernel, which fits 4 waves per SIMD.
-  const bool shallow = a.max_depth + 1 <= kShallowFrames;
+  // The stack needs one frame per level below a walk's start. The first
+  // kShallowFrames frames are in LDS (4 waves per SIMD); trees with more
+  // levels (any system of a batch) also use the global deep frames.
+  const bool deep = a.max_depth + 1 > kShallowFrames;
   if (!any_unsoftened) {
-    if (shallow)
-      hipLaunchKernelGGL((k_walk<Real, kWaves, false, kShallowFrames>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+    if (!deep)
+      hipLaunchKernelGGL((k_walk<Real, kWaves, false, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
     else
-      hipLaunchKernelGGL((k_walk<Real, kWaves, false, kLevels>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+      hipLaunchKernelGGL((k_walk<Real, kWaves, false, true>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
   } else {
-    if (shallow)
-      hipLaunchKernelGGL((k_walk<Real, kWaves, true, kShallowFrames>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+    if (!deep)
+      hipLaunchKernelGGL((k_walk<Real, kWaves, true, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
     else
-      hipLaunchKernelGGL((k_walk<Real, kWaves, true, kLevels>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
+      hipLaunchKernelGGL((k_walk<Real, kWaves, true, true>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
   }
   return hipGetLastError();
 }
