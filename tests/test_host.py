@@ -46,7 +46,7 @@ def test_server_footprint_model():
     lib = bl.load()
     f16 = lib.bh_server_footprint(bl.ctypes.byref(bl.request(1 << 20, max_leaf=16)))
     f8 = lib.bh_server_footprint(bl.ctypes.byref(bl.request(1 << 20, max_leaf=8)))
-    assert 200 * (1 << 20) < f16 < f8 < 1000 * (1 << 20)
+    assert 350 * (1 << 20) < f16 < f8 < 1000 * (1 << 20)
 
 
 def test_bench_refuses_several_gpus():
