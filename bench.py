@@ -50,7 +50,8 @@ def rel_errors(test, ref):
 
 
 def accuracy(ctx, p, fp, args):
-    """FP32 tree accelerations vs the FP64 direct sum on a seeded subset."""
+    """FP32 tree accelerations vs the FP64 direct sum on a seeded subset; also
+    returns the accelerations (Morton order) for the tree-force comparison."""
     ctx.build_tree(p, args.max_leaf, args.group_size)
     acc, _, _, _ = ctx.accel(fp, args.target_size)
     perm = ctx.export_perm()  # Morton position -> input index
@@ -60,7 +61,7 @@ def accuracy(ctx, p, fp, args):
     where[perm] = np.arange(len(p))
     ref = ctx.direct_sum(p, idx, fp["eps2"][0], fp["G"][0])
     rms, p99, glob = rel_errors(acc[where[idx]], ref)
-    return {"rms_rel": rms, "p99_rel": p99, "global_rel": glob, "subset": int(len(idx)), "reference": "FP64 direct sum"}
+    return {"rms_rel": rms, "p99_rel": p99, "global_rel": glob, "subset": int(len(idx)), "reference": "FP64 direct sum"}, acc
 
 
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_walk_plummer16m.md")
@@ -80,7 +81,10 @@ def profiled_traffic(args):
     return None
 
 
-def cpu_baseline(p, fp, args):
+def cpu_baseline(p, fp, args, gpu_acc=None):
+    """The FP64 oracle on the host's cores over a bounded sample; with the
+    GPU's accelerations of the same tree (Morton order) also returns their
+    error against the oracle's FP64 tree forces on the sampled groups."""
     sys.path.insert(0, REPO)
     import oracle  # the FP64 restatement, timed as the CPU baseline
 
@@ -90,10 +94,16 @@ def cpu_baseline(p, fp, args):
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
     o = oracle.Oracle()
-    build_ms, walk_ms, frac = o.baseline_sample(p, fp, args.max_leaf, args.group_size, args.target_size,
-                                                args.cpu_stride, cores)
+    (build_ms, walk_ms, frac), oacc, mask = o.baseline_sample(p, fp, args.max_leaf, args.group_size,
+                                                              args.target_size, args.cpu_stride, cores, with_acc=True)
+    tree_err = None
+    if gpu_acc is not None:
+        rms, p99, glob = rel_errors(gpu_acc[mask], oacc[mask])
+        tree_err = {"rms_rel": rms, "p99_rel": p99, "global_rel": glob, "particles": int(mask.sum()),
+                    "reference": "FP64 grouped walk (oracle) on the same tree, every "
+                                 f"{args.cpu_stride}th group"}
     step_s = build_ms / 1e3 + (walk_ms / 1e3) / frac
-    return {
+    return tree_err, {
         "value": len(p) / step_s,
         "unit": "particles/s",
         "cores": cores,
@@ -349,7 +359,7 @@ def main():
     ctx = bl.Context(0)
     ctx.set_precision(bl.BH_FP64 if args.fp64 else bl.BH_FP32)
 
-    acc_report = accuracy(ctx, p, fp, args)
+    acc_report, gpu_acc = accuracy(ctx, p, fp, args)
     log(f"accuracy vs FP64 direct sum: {acc_report}")
 
     ctx.state_upload(p)
@@ -426,7 +436,9 @@ def main():
     if not args.no_cpu_baseline:
         if p is None:
             p = ic.plummer(args.n, args.seed)
-        out["cpu_baseline"] = cpu_baseline(p, fp, args)
+        tree_err, out["cpu_baseline"] = cpu_baseline(p, fp, args, gpu_acc)
+        out["accuracy_vs_tree_fp64"] = tree_err
+        log(f"accuracy vs the FP64 tree forces: {tree_err}")
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -113,7 +113,7 @@ class Oracle:
         lib.bho_targets.argtypes = [_P, _U32, _P, _U64]
         lib.bho_targets.restype = _U64
         lib.bho_step.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _D, _D, _I, _P]
-        lib.bho_baseline_sample.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _U32, _I, _P]
+        lib.bho_baseline_sample.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _U32, _I, _P, _P, _P]
         lib.bho_brute_force.argtypes = [_P, _U64, _D, _D, _I, _P]
         lib.bho_brute_subset.argtypes = [_P, _U64, _P, _U64, _D, _D, _I, _P]
         lib.bho_accel_from_source.argtypes = [_P, _D, _P, _P, _D, _I, _D, _P]
@@ -155,13 +155,18 @@ class Oracle:
             raise ValueError("oracle step rejected the input")
         return out, timing
 
-    def baseline_sample(self, p, fp, max_leaf, group_size, target_size, stride, threads=0):
+    def baseline_sample(self, p, fp, max_leaf, group_size, target_size, stride, threads=0, with_acc=False):
+        """(build ms, walk ms, covered fraction) of a full build plus the walk of
+        every stride-th group; with_acc also returns the sampled groups'
+        accelerations (Morton order) and the mask of sampled particles."""
         out = np.zeros(3)
+        acc = np.zeros((len(p), 3)) if with_acc else None
+        mask = np.zeros(len(p), np.uint8) if with_acc else None
         rc = self.lib.bho_baseline_sample(_ptr(p), len(p), _ptr(fp), max_leaf, group_size, target_size, stride, threads,
-                                          _ptr(out))
+                                          _ptr(out), _ptr(acc), _ptr(mask))
         if rc != 0:
             raise ValueError("baseline sample rejected the input")
-        return out
+        return (out, acc, mask.astype(bool)) if with_acc else out
 
     def brute_force(self, p, eps2, G=1.0, threads=0):
         acc = np.zeros((len(p), 3))

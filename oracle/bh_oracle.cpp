@@ -888,7 +888,8 @@ int bho_step(Particle* p, uint64_t n, const ForceParams* fp, uint32_t max_leaf, 
 // walk over every `stride`-th group only (no kick/drift). out3: build_ms,
 // walk_ms over the sampled groups, fraction of particles those groups hold.
 int bho_baseline_sample(const Particle* p, uint64_t n, const ForceParams* fp, uint32_t max_leaf, uint32_t group_size,
-                        uint32_t target_size, uint32_t stride, int threads, double* out3) {
+                        uint32_t target_size, uint32_t stride, int threads, double* out3, double* acc_out,
+                        uint8_t* sampled) {
   if (n == 0 || n >= (1ull << 31) || group_size == 0 || stride == 0) return 1;
   const int th = hw_threads(threads);
   using clock = std::chrono::steady_clock;
@@ -915,6 +916,15 @@ int bho_baseline_sample(const Particle* p, uint64_t n, const ForceParams* fp, ui
   out3[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
   out3[1] = std::chrono::duration<double, std::milli>(t2 - t1).count();
   out3[2] = static_cast<double>(covered) / static_cast<double>(n);
+  // the sampled groups' accelerations, Morton order (optional)
+  if (acc_out) std::copy(acc.begin(), acc.end(), acc_out);
+  if (sampled) {
+    std::fill(sampled, sampled + n, uint8_t(0));
+    for (uint32_t g : sample) {
+      const auto& nd = t.nodes[g];
+      std::fill(sampled + nd.particle_begin, sampled + nd.particle_begin + nd.particle_count, uint8_t(1));
+    }
+  }
   return 0;
 }
 

@@ -319,3 +319,16 @@ def test_rejects_bad_input(orc):
     p["pos"][5, 1] = np.nan
     with pytest.raises(ValueError):
         orc.build(p, 8, 64)
+
+
+def test_baseline_sample_accelerations_match_grouped_pass(orc):
+    """The CPU baseline's sampled groups get exactly the grouped pass's
+    accelerations (same tree, same walker)."""
+    from bhgpu import ic, layout
+
+    p = ic.plummer(20000, 4)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    (build_ms, walk_ms, frac), acc, mask = orc.baseline_sample(p, fp, 16, 256, 32, 7, 4, with_acc=True)
+    full, _, _, _ = orc.build(p, 16, 256).grouped_pass(fp, 32)
+    assert 0.05 < frac < 0.3 and abs(mask.mean() - frac) < 1e-12
+    assert np.array_equal(acc[mask], full[mask]) and not acc[~mask].any()
