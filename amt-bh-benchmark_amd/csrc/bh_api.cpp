@@ -61,10 +61,9 @@ struct Ctx {
   // tree (level-order BFS)
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
   DevBuf<uint8_t> depth, leaf, nchild;
-  DevBuf<double4> ctr, mac;
+  DevBuf<double4> ctr;
+  DevBuf<NodeRec> rec;
   DevBuf<double> mass, com, iab, q64;
-  DevBuf<uint2> link;
-  DevBuf<float4> q32;
   // direct sum
   DevBuf<double4> dsrc;
   DevBuf<uint32_t> didx;
@@ -185,9 +184,7 @@ TreeDev tree_view(Ctx* c) {
   t.mass = c->mass.p;
   t.com = c->com.p;
   t.iab = c->iab.p;
-  t.mac = c->mac.p;
-  t.link = c->link.p;
-  t.q32 = c->q32.p;
+  t.rec = c->rec.p;
   t.q64 = c->q64.p;
   t.leaf_node = c->leaf_node.p;
   t.leaf_first = c->leaf_first.p;
@@ -361,13 +358,12 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n) {
   TRY(ensure(c, c->leaf, m, "nodes"));
   TRY(ensure(c, c->nchild, m, "nodes"));
   TRY(ensure(c, c->ctr, m, "nodes"));
-  TRY(ensure(c, c->mac, m, "nodes"));
+  TRY(ensure(c, c->rec, m, "nodes"));
   TRY(ensure(c, c->mass, m, "nodes"));
   TRY(ensure(c, c->com, 3ull * m, "nodes"));
   TRY(ensure(c, c->iab, 6ull * m, "nodes"));
   TRY(ensure(c, c->q64, 8ull * m, "nodes"));
-  TRY(ensure(c, c->link, m, "nodes"));
-  TRY(ensure(c, c->q32, 2ull * m, "nodes"));
+
   TRY(ensure(c, c->leaf_node, leaves, "leaves"));
   TRY(ensure(c, c->leaf_first, leaves, "leaves"));
   const TreeDev t = tree_view(c);
@@ -409,9 +405,7 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   const uint32_t n_groups = c->host_misc[41];
   TRY(ensure(c, c->defer, uint64_t(n_groups) * kDefer, "deferred nodes"));
   WalkArgs a;
-  a.mac = c->mac.p;
-  a.link = c->link.p;
-  a.q32 = c->q32.p;
+  a.rec = c->rec.p;
   a.q64 = c->q64.p;
   a.ctr = c->ctr.p;
   a.begin = c->begin.p;
@@ -622,13 +616,12 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->leaf);
   release(c->nchild);
   release(c->ctr);
-  release(c->mac);
+  release(c->rec);
   release(c->mass);
   release(c->com);
   release(c->iab);
   release(c->q64);
-  release(c->link);
-  release(c->q32);
+
   release(c->dsrc);
   release(c->didx);
   release(c->dpart);

@@ -49,6 +49,17 @@ constexpr uint32_t kShallowFrames = 12;  // walk stack of trees with at most thi
 // The device a context was created on.
 int ctx_device(const bh_ctx* ctx);
 
+// What the walk reads of a node, in one 64-byte record: the MAC data (FP64
+// com and squared edge), the link (leaf: first particle and count | 1 << 31;
+// interior: first child and child count) and the FP32 mass and traceless
+// quadrupole (Qzz = -(Qxx + Qyy)).
+struct alignas(64) NodeRec {
+  double cx, cy, cz, edge2;
+  uint32_t first, count;
+  float m, qxx, qyy, qxy, qxz, qyz;
+};
+static_assert(sizeof(NodeRec) == 64, "NodeRec layout");
+
 // Device view of one tree (level-order BFS numbering).
 struct TreeDev {
   uint32_t* begin;       // [M] first particle (Morton position)
@@ -64,9 +75,7 @@ struct TreeDev {
   double* mass;          // [M]
   double* com;           // [3M]
   double* iab;           // [6M] xx,yy,zz,xy,xz,yz about the com
-  double4* mac;          // [M] com xyz, (2*hw)^2
-  uint2* link;           // [M] leaf: (particle_begin, count | 1<<31); interior: (first_child, nchild)
-  float4* q32;           // [2M] {mass, Qxx, Qyy, Qzz}, {Qxy, Qxz, Qyz, 0}
+  NodeRec* rec;          // [M] the walk's node record
   double* q64;           // [8M] mass, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, 0
   uint32_t* leaf_node;   // [leaves] node of each leaf, Morton order
   uint32_t* leaf_first;  // [leaves] first particle of each leaf
@@ -74,9 +83,7 @@ struct TreeDev {
 };
 
 struct WalkArgs {
-  const double4* mac;
-  const uint2* link;
-  const float4* q32;
+  const NodeRec* rec;
   const double* q64;
   const double4* ctr;
   const uint32_t* begin;

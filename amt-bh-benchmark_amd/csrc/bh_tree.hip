@@ -335,17 +335,26 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
     for (int a = 0; a < 6; ++a) t.iab[6 * i + a] = ib[a];
     // walk records: MAC data, links, traceless quadrupole (multipole.h:185-188)
     const double edge = 2.0 * cc.w;
-    t.mac[i] = make_double4(com[0], com[1], com[2], edge * edge);
-    t.link[i] = leaf ? make_uint2(t.begin[i], cnt | 0x80000000u) : make_uint2(t.first_child[i], t.nchild[i]);
     const double tr = ib[0] + ib[1] + ib[2];
     const double q[6] = {3.0 * ib[0] - tr, 3.0 * ib[1] - tr, 3.0 * ib[2] - tr, 3.0 * ib[3], 3.0 * ib[4], 3.0 * ib[5]};
     double* q64 = &t.q64[8 * i];
     q64[0] = rm;
     for (int a = 0; a < 6; ++a) q64[1 + a] = q[a];
     q64[7] = 0.0;
-    t.q32[2 * i + 0] = make_float4(static_cast<float>(rm), static_cast<float>(q[0]), static_cast<float>(q[1]),
-                                   static_cast<float>(q[2]));
-    t.q32[2 * i + 1] = make_float4(static_cast<float>(q[3]), static_cast<float>(q[4]), static_cast<float>(q[5]), 0.0f);
+    NodeRec r;
+    r.cx = com[0];
+    r.cy = com[1];
+    r.cz = com[2];
+    r.edge2 = edge * edge;
+    r.first = leaf ? t.begin[i] : t.first_child[i];
+    r.count = leaf ? (cnt | 0x80000000u) : static_cast<uint32_t>(t.nchild[i]);
+    r.m = static_cast<float>(rm);
+    r.qxx = static_cast<float>(q[0]);
+    r.qyy = static_cast<float>(q[1]);
+    r.qxy = static_cast<float>(q[3]);
+    r.qxz = static_cast<float>(q[4]);
+    r.qyz = static_cast<float>(q[5]);
+    t.rec[i] = r;
   }
 }
 

@@ -471,9 +471,10 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
     uint32_t lfirst = 0, lcnt = 0;
     double4 mac = make_double4(0, 0, 0, 0);
     const uint32_t nd = node < last_node ? node : last_node;
+    const NodeRec* rp = &a.rec[nd];
     if (nv) {
-      mac = a.mac[nd];
-      const uint2 lk = a.link[nd];
+      mac = make_double4(rp->cx, rp->cy, rp->cz, rp->edge2);
+      const uint2 lk = make_uint2(rp->first, rp->count);
       const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
       const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
       const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
@@ -537,10 +538,10 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
         if (cellf) {
           const uint32_t slot = pc_cnt + pos;
           if constexpr (sizeof(Real) == 4) {
-            const float4 q0 = a.q32[2 * nd + 0], q1 = a.q32[2 * nd + 1];
-            L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q0.x);
-            L.pc.b[slot] = make_float4(q0.y, q0.z, q0.w, q1.x);
-            L.pc.c[slot] = make_float2(q1.y, q1.z);
+            const float qxx = rp->qxx, qyy = rp->qyy;
+            L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, rp->m);
+            L.pc.b[slot] = make_float4(qxx, qyy, -(qxx + qyy), rp->qxy);
+            L.pc.c[slot] = make_float2(rp->qxz, rp->qyz);
           } else {
             const double* q = &a.q64[8 * nd];
             L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q[0]);
@@ -553,7 +554,7 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
         while (pc_cnt >= B) flush_pc();
       } else {
         if (cellf) {
-          const double m = sizeof(Real) == 4 ? static_cast<double>(a.q32[2 * nd].x) : a.q64[8 * nd];
+          const double m = sizeof(Real) == 4 ? static_cast<double>(rp->m) : a.q64[8 * nd];
           L.pp[pp_cnt + pos] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, m);
         }
         pp_cnt += nc;
@@ -744,7 +745,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   const uint32_t G = uniform(a.count[g]);
   const uint32_t sys_id = uniform(a.sid[gb]);
   const SysDev& sp = a.sys[sys_id];
-  const bool gleaf = (a.link[g].y >> 31) != 0;
+  const bool gleaf = (a.rec[g].count >> 31) != 0;
   uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(unit) * (kLevels - Lds::kFrames) * kWave : nullptr;
   if (gleaf || G <= sp.tsm || G > kGroupCap) {
     walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0);
