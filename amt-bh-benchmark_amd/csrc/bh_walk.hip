@@ -165,6 +165,13 @@ __device__ inline uint64_t lanemask_lt() {
 }
 
 __device__ inline uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// A wave-uniform double, held in scalar registers.
+__device__ inline double uniform_d(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b >> 32));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
 
 template <typename Real>
 __device__ inline typename Vec4T<Real>::T make_entry(double x, double y, double z, double m) {
@@ -374,7 +381,12 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
   const uint64_t lt = lanemask_lt();
   const uint32_t last_node = a.n_nodes - 1, last_part = a.n_parts - 1;
   const uint32_t B = ev.B;
-  const double blo0 = box[0], blo1 = box[1], blo2 = box[2], bhi0 = box[3], bhi1 = box[4], bhi2 = box[5];
+  const double blo0 = uniform_d(box[0]), blo1 = uniform_d(box[1]), blo2 = uniform_d(box[2]);
+  const double bhi0 = uniform_d(box[3]), bhi1 = uniform_d(box[4]), bhi2 = uniform_d(box[5]);
+  cx = uniform_d(cx);
+  cy = uniform_d(cy);
+  cz = uniform_d(cz);
+  theta2 = uniform_d(theta2);
   uint32_t pp_cnt = 0, pc_cnt = 0;
   bool ok = true;
 
