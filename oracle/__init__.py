@@ -83,6 +83,16 @@ class OracleTree:
         """The same from independent per-target walks from the root."""
         return self._walk(self._lib.bho_target_walk, fp, target_size, threads)
 
+    def per_node_pass(self, fp, threads=0):
+        """The reference's per-node path (every leaf a target, recursive
+        descent per target particle): accelerations, direct-source and cell
+        counts per particle, Morton order."""
+        acc = np.zeros((self.n, 3))
+        npp = np.zeros(self.n, np.uint32)
+        npc = np.zeros(self.n, np.uint32)
+        self._lib.bho_per_node_pass(self._h, _ptr(fp), threads, _ptr(acc), _ptr(npp), _ptr(npc))
+        return acc, npp, npc
+
     def targets(self, target_size):
         cap = self.n
         out = np.zeros(cap, np.uint32)
@@ -113,6 +123,7 @@ class Oracle:
         lib.bho_targets.argtypes = [_P, _U32, _P, _U64]
         lib.bho_targets.restype = _U64
         lib.bho_step.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _D, _D, _I, _P]
+        lib.bho_per_node_pass.argtypes = [_P, _P, _I, _P, _P, _P]
         lib.bho_baseline_sample.argtypes = [_P, _U64, _P, _U32, _U32, _U32, _U32, _I, _P, _P, _P]
         lib.bho_brute_force.argtypes = [_P, _U64, _D, _D, _I, _P]
         lib.bho_brute_subset.argtypes = [_P, _U64, _P, _U64, _D, _D, _I, _P]

@@ -843,6 +843,72 @@ int bho_target_walk(const bho_tree* h, const ForceParams* fp, uint32_t target_si
   return 0;
 }
 
+// The per-node TTG path (traversal_ttg.h:156-196) restated serially: every
+// leaf is a target; for each (target leaf, source node), starting at the
+// root: the MAC on the target leaf's particle box (never for the target
+// itself) accepts the source's multipole; a leaf source not accepted is
+// summed directly, skipping the particle itself by id; any other source is
+// descended into its children. Contributions are summed per source subtree
+// in child order (the reference's aggregator order is unspecified).
+// Per-particle counts: direct sources (n_pp) and accepted cells (n_pc).
+int bho_per_node_pass(const bho_tree* h, const ForceParams* fp, int threads, double* acc, uint32_t* n_pp,
+                      uint32_t* n_pc) {
+  const Tree* t = reinterpret_cast<const Tree*>(h);
+  std::vector<uint32_t> leaves;
+  for (uint32_t i = 0; i < t->nodes.size(); ++i)
+    if (t->nodes[i].leaf) leaves.push_back(i);
+  parallel_for(leaves.size(), hw_threads(threads), [&](uint64_t k) {
+    const uint32_t tl = leaves[k];
+    const WalkNode& tn = t->nodes[tl];
+    BBox box;
+    for (uint32_t i = tn.particle_begin; i < tn.particle_begin + tn.particle_count; ++i)
+      box.add(t->x[i], t->y[i], t->z[i]);
+    for (uint32_t i = tn.particle_begin; i < tn.particle_begin + tn.particle_count; ++i) {
+      const Particle& tp = t->particles[i];
+      uint32_t pp = 0, pc = 0;
+      // recursive descent, children in order (source-subtree sums)
+      auto visit = [&](auto&& self, uint32_t s) -> std::array<double, 3> {
+        const WalkNode& sn = t->nodes[s];
+        std::array<double, 3> a{0.0, 0.0, 0.0};
+        if (s != tl && mac_accept(box, sn.com, 2.0 * sn.half_width, fp->theta)) {
+          double o[3];
+          accel_from_source(tp.pos, sn.mass, sn.com, t->iab[s], fp->eps2, fp->use_quadrupole != 0, fp->G, o);
+          ++pc;
+          return {o[0], o[1], o[2]};
+        }
+        if (sn.leaf) {
+          static const Sym3 zero{};
+          for (uint32_t j = sn.particle_begin; j < sn.particle_begin + sn.particle_count; ++j) {
+            const Particle& sp = t->particles[j];
+            ++pp;
+            if (sp.id == tp.id) continue;
+            double o[3];
+            accel_from_source(tp.pos, sp.mass, sp.pos, zero, fp->eps2, false, fp->G, o);
+            a[0] += o[0];
+            a[1] += o[1];
+            a[2] += o[2];
+          }
+          return a;
+        }
+        for (uint32_t c = 0; c < sn.num_children; ++c) {
+          const auto b = self(self, sn.first_child + c);
+          a[0] += b[0];
+          a[1] += b[1];
+          a[2] += b[2];
+        }
+        return a;
+      };
+      const auto a = visit(visit, 0u);
+      acc[3 * i + 0] = a[0];
+      acc[3 * i + 1] = a[1];
+      acc[3 * i + 2] = a[2];
+      if (n_pp) n_pp[i] = pp;
+      if (n_pc) n_pc[i] = pc;
+    }
+  });
+  return 0;
+}
+
 // Target node ids in Morton order (topmost with leaf || count <= target_size
 // below the group roots); returns the count, fills up to cap entries.
 uint64_t bho_targets(const bho_tree* h, uint32_t target_size, uint32_t* out, uint64_t cap) {

@@ -123,6 +123,25 @@ def test_walk_modes_clustered(gpu, orc, mode, theta, kind):
         gpu.set_precision(bl.BH_FP32)
 
 
+@pytest.mark.parametrize("theta,quad", [(0.5, True), (0.7, True), (0.5, False)])
+def test_per_node_cross_check(gpu, orc, theta, quad):
+    """target_size 0 makes every leaf a target: the lists of the reference's
+    per-node path (traversal_ttg.h:156-196), restated in the oracle."""
+    p = ic.hernquist(20000, 9)
+    fp = layout.force_params(theta, 1e-4, 1.0, quad)
+    oa, opp, opc = orc.build(p, 8, 64).per_node_pass(fp)
+    if not quad:
+        opp, opc = opp + opc, opc * 0
+    gpu.build_tree(p, 8, 64)
+    for prec in (bl.BH_FP64, bl.BH_FP32):
+        print(f"per-node cross-check theta={theta} quad={quad} prec={prec}", flush=True)
+        gpu.set_precision(prec)
+        ga, gpp, gpc, _ = gpu.accel(fp, 0)
+        assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc)
+        assert rel_err(ga, oa) < TOL[prec]
+    gpu.set_precision(bl.BH_FP32)
+
+
 def test_large_targets_span_chunks(gpu, orc):
     # targets of more than 64 particles (target_size 200 and a 150-fold duplicate leaf)
     p = ic.random_box(4000, 12, 10.0)

@@ -332,3 +332,24 @@ def test_baseline_sample_accelerations_match_grouped_pass(orc):
     full, _, _, _ = orc.build(p, 16, 256).grouped_pass(fp, 32)
     assert 0.05 < frac < 0.3 and abs(mask.mean() - frac) < 1e-12
     assert np.array_equal(acc[mask], full[mask]) and not acc[~mask].any()
+
+
+def test_per_node_path_equals_grouped_with_leaf_targets(orc):
+    """The per-node path (traversal_ttg.h:156-196: every leaf a target, MAC on
+    the leaf's particle box, direct sums over unaccepted leaves) has exactly
+    the interaction lists of the grouped walk with target_size 0 (targets =
+    leaves), so the same forces up to summation order (eps2 > 0: the self
+    term is zero either way)."""
+    from bhgpu import ic, layout
+
+    p = ic.plummer(6000, 8)
+    for theta, quad in ((0.5, True), (0.7, False), (0.0, True)):
+        fp = layout.force_params(theta, 1e-4, 1.0, quad)
+        t = orc.build(p, 8, 64)
+        a, pp, pc = t.per_node_pass(fp)
+        g, gpp, gpc, _ = t.grouped_pass(fp, 0)
+        if quad:
+            assert np.array_equal(pp, gpp) and np.array_equal(pc, gpc)
+        else:  # without quadrupoles accepted cells are monopoles in the P2P list
+            assert np.array_equal(pp + pc, gpp)
+        assert np.sqrt(((a - g) ** 2).sum() / (g ** 2).sum()) < 1e-13
