@@ -50,7 +50,8 @@ namespace {
 
 constexpr int kBatch = 48;  // entries evaluated per batch (at most)
 constexpr int kBuf = 112;   // staging capacity: kBatch - 1 left over + 64 staged at once
-constexpr int kGroupChunks = kGroupCap / kWave;  // a group's particles per lane
+constexpr int kGroupChunks = kGroupCap / kWave;  // a group's particle slots per lane
+static_assert(kBatch == 48, "the group layouts assume 48-entry batches");
 
 template <typename Real>
 struct Vec4T;
@@ -211,36 +212,35 @@ __device__ inline void eval_pp(const typename Vec4T<Real>::T* e, int C, Real tx,
   acc.add(bx, by, bz);
 }
 
-// Monopole + traceless quadrupole over this lane's C consecutive cells
-// (multipole.h:171-200): a = M d / r^3 - Q.d / r^5 + 2.5 (d.Q.d) d / r^7.
+// Monopole + traceless quadrupole of one cell (multipole.h:171-200):
+// a = M d / r^3 - Q.d / r^5 + 2.5 (d.Q.d) d / r^7, evaluated as
+// r^-5 (d (M r^2 + 2.5 (d.Q.d) / r^2) - Q.d) with the softened r^2.
+template <typename V4, typename V2, typename Real>
+__device__ inline void p2c_term(const V4& e0, const V4& e1, const V2& e2, Real tx, Real ty, Real tz, Real eps2,
+                                Real& bx, Real& by, Real& bz) {
+  const Real dx = e0.x - tx, dy = e0.y - ty, dz = e0.z - tz;
+  const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
+  const Real inv = rsq(r2);
+  const Real inv2 = inv * inv;
+  const Real inv5 = (inv * inv2) * inv2;
+  // Q.d with Q = {xx, yy, zz, xy, xz, yz} = {e1.x, e1.y, e1.z, e1.w, e2.x, e2.y}
+  const Real qx = fmaR(e1.x, dx, fmaR(e1.w, dy, e2.x * dz));
+  const Real qy = fmaR(e1.w, dx, fmaR(e1.y, dy, e2.y * dz));
+  const Real qz = fmaR(e2.x, dx, fmaR(e2.y, dy, e1.z * dz));
+  const Real dqd = fmaR(dx, qx, fmaR(dy, qy, dz * qz));
+  const Real g = fmaR(Real(2.5) * dqd, inv2, e0.w * r2);
+  bx = fmaR(inv5, fmaR(g, dx, -qx), bx);
+  by = fmaR(inv5, fmaR(g, dy, -qy), by);
+  bz = fmaR(inv5, fmaR(g, dz, -qz), bz);
+}
+
+// P2C over this lane's C consecutive cells from j0.
 template <typename Real>
 __device__ inline void eval_pc(const PcRing<Real>& r, int j0, int C, Real tx, Real ty, Real tz, Real eps2,
                                Acc<Real>& acc) {
-  using V4 = typename Vec4T<Real>::T;
-  using V2 = typename Vec2T<Real>::T;
   Real bx = 0, by = 0, bz = 0;
 #pragma unroll 2
-  for (int k = 0; k < C; ++k) {
-    const V4 e0 = r.a[j0 + k];
-    const V4 e1 = r.b[j0 + k];
-    const V2 e2 = r.c[j0 + k];
-    const Real dx = e0.x - tx, dy = e0.y - ty, dz = e0.z - tz;
-    const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
-    const Real inv = rsq(r2);
-    const Real inv2 = inv * inv;
-    const Real inv3 = inv * inv2;
-    // Q.d with Q = {xx, yy, zz, xy, xz, yz} = {e1.x, e1.y, e1.z, e1.w, e2.x, e2.y}
-    const Real qx = fmaR(e1.x, dx, fmaR(e1.w, dy, e2.x * dz));
-    const Real qy = fmaR(e1.w, dx, fmaR(e1.y, dy, e2.y * dz));
-    const Real qz = fmaR(e2.x, dx, fmaR(e2.y, dy, e1.z * dz));
-    const Real dqd = fmaR(dx, qx, fmaR(dy, qy, dz * qz));
-    const Real inv5 = inv3 * inv2;
-    const Real inv7 = inv5 * inv2;
-    const Real f = fmaR(Real(2.5) * dqd, inv7, e0.w * inv3);
-    bx = fmaR(f, dx, fmaR(-inv5, qx, bx));
-    by = fmaR(f, dy, fmaR(-inv5, qy, by));
-    bz = fmaR(f, dz, fmaR(-inv5, qz, bz));
-  }
+  for (int k = 0; k < C; ++k) p2c_term(r.a[j0 + k], r.b[j0 + k], r.c[j0 + k], tx, ty, tz, eps2, bx, by, bz);
   acc.add(bx, by, bz);
 }
 
@@ -253,17 +253,17 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// Group-phase P2P / P2C: every lane holds NC (<= 4) particles of the group,
-// lane + 64 k, and takes all 64 entries of the batch (LDS broadcast reads).
+// Group-phase P2P / P2C: every lane holds NC (<= 2) particles of the group
+// and takes the C entries at e (the same entries for both).
 template <typename Real, bool kMask, int NC>
-__device__ inline void eval_pp_group(const typename Vec4T<Real>::T* e, const Real* gx, const Real* gy, const Real* gz,
-                                     Real eps2, Acc<Real>* acc) {
+__device__ inline void eval_pp_group(const typename Vec4T<Real>::T* e, int C, const Real* gx, const Real* gy,
+                                     const Real* gz, Real eps2, Acc<Real>* acc) {
   using V4 = typename Vec4T<Real>::T;
   Real bx[NC], by[NC], bz[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
 #pragma unroll 2
-  for (int j = 0; j < kBatch; ++j) {
+  for (int j = 0; j < C; ++j) {
     const V4 q = e[j];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -282,35 +282,19 @@ __device__ inline void eval_pp_group(const typename Vec4T<Real>::T* e, const Rea
 }
 
 template <typename Real, int NC>
-__device__ inline void eval_pc_group(const PcRing<Real>& r, const Real* gx, const Real* gy, const Real* gz,
-                                     Real eps2, Acc<Real>* acc) {
+__device__ inline void eval_pc_group(const PcRing<Real>& r, int j0, int C, const Real* gx, const Real* gy,
+                                     const Real* gz, Real eps2, Acc<Real>* acc) {
   using V4 = typename Vec4T<Real>::T;
   using V2 = typename Vec2T<Real>::T;
   Real bx[NC], by[NC], bz[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) bx[k] = by[k] = bz[k] = 0;
-  for (int j = 0; j < kBatch; ++j) {
-    const V4 e0 = r.a[j];
-    const V4 e1 = r.b[j];
-    const V2 e2 = r.c[j];
+  for (int k2 = 0; k2 < C; ++k2) {
+    const V4 e0 = r.a[j0 + k2];
+    const V4 e1 = r.b[j0 + k2];
+    const V2 e2 = r.c[j0 + k2];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      const Real dx = e0.x - gx[k], dy = e0.y - gy[k], dz = e0.z - gz[k];
-      const Real r2 = fmaR(dx, dx, fmaR(dy, dy, fmaR(dz, dz, eps2)));
-      const Real inv = rsq(r2);
-      const Real inv2 = inv * inv;
-      const Real inv3 = inv * inv2;
-      const Real qx = fmaR(e1.x, dx, fmaR(e1.w, dy, e2.x * dz));
-      const Real qy = fmaR(e1.w, dx, fmaR(e1.y, dy, e2.y * dz));
-      const Real qz = fmaR(e2.x, dx, fmaR(e2.y, dy, e1.z * dz));
-      const Real dqd = fmaR(dx, qx, fmaR(dy, qy, dz * qz));
-      const Real inv5 = inv3 * inv2;
-      const Real inv7 = inv5 * inv2;
-      const Real f = fmaR(Real(2.5) * dqd, inv7, e0.w * inv3);
-      bx[k] = fmaR(f, dx, fmaR(-inv5, qx, bx[k]));
-      by[k] = fmaR(f, dy, fmaR(-inv5, qy, by[k]));
-      bz[k] = fmaR(f, dz, fmaR(-inv5, qz, bz[k]));
-    }
+    for (int k = 0; k < NC; ++k) p2c_term(e0, e1, e2, gx[k], gy[k], gz[k], eps2, bx[k], by[k], bz[k]);
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) acc[k].add(bx[k], by[k], bz[k]);
@@ -334,28 +318,94 @@ struct TargetEval {
   }
 };
 
-// Evaluator of a whole group (<= 256 particles): lane + 64 k, batches of 64.
+// Evaluator of a whole group (G <= 128 particles, batches of kBatch
+// entries). Each lane has two particle slots; slot k of lane l holds
+// particle base_k + l % P_k and takes slice l / P_k of S_k slices (C_k =
+// kBatch / S_k consecutive entries of each batch). Layouts, by G:
+//   G <= 42      two slots, P = ceil(G / 2), S = 3 (one fused loop)
+//   43 .. 64     one slot, P = 64, S = 1
+//   65 .. 96     slot 0: particles 0..63, every entry; slot 1: the r = G - 64
+//                others, S_1 = the largest divisor of kBatch with r S_1 <= 64
+//   97 .. 128    two slots, P = 64, S = 1 (one fused loop)
+// Slices are summed in FP64 at the end.
 template <typename Real, bool kMask>
 struct GroupEval {
   using V4 = typename Vec4T<Real>::T;
   Real gx[kGroupChunks], gy[kGroupChunks], gz[kGroupChunks], eps2;
-  int nc;
+  int nc;          // slots in the fused loop (1 or 2)
+  int c0, o0;      // fused loop: entries [o0, o0 + c0) of each batch
+  int c1, o1;      // tail slot 1 (c1 > 0): entries [o1, o1 + c1)
+  bool tail_on;    // this lane's tail slot holds a particle
+  uint32_t base[kGroupChunks], rows[kGroupChunks], slices[kGroupChunks];
   uint32_t B;
   Acc<Real> acc[kGroupChunks];
+
+  __device__ inline void setup(uint32_t G, int lane) {
+    const uint32_t l = static_cast<uint32_t>(lane);
+    c1 = 0;
+    o1 = 0;
+    tail_on = false;
+    if (G <= 42) {
+      const uint32_t P = (G + 1) / 2;
+      nc = 2;
+      base[0] = 0;
+      base[1] = P;
+      rows[0] = rows[1] = P;
+      slices[0] = slices[1] = 3;
+    } else if (G <= 64) {
+      nc = 1;
+      base[0] = base[1] = 0;
+      rows[0] = rows[1] = kWave;
+      slices[0] = slices[1] = 1;
+    } else if (G <= 96) {
+      const uint32_t r = G - kWave;
+      constexpr uint32_t kDiv[9] = {48u, 24u, 16u, 12u, 8u, 6u, 4u, 3u, 2u};  // divisors of kBatch
+      uint32_t S1 = 1;
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        if (S1 == 1 && kDiv[q] * r <= static_cast<uint32_t>(kWave)) S1 = kDiv[q];
+      nc = 1;
+      base[0] = 0;
+      base[1] = kWave;
+      rows[0] = kWave;
+      rows[1] = r;
+      slices[0] = 1;
+      slices[1] = S1;
+      c1 = kBatch / static_cast<int>(S1);
+      const uint32_t s1 = l / r < S1 ? l / r : S1 - 1;
+      o1 = static_cast<int>(s1) * c1;
+      tail_on = l < r * S1;
+    } else {
+      nc = 2;
+      base[0] = 0;
+      base[1] = kWave;
+      rows[0] = rows[1] = kWave;
+      slices[0] = slices[1] = 1;
+    }
+    c0 = kBatch / static_cast<int>(slices[0]);
+    const uint32_t s0 = l / rows[0] < slices[0] ? l / rows[0] : slices[0] - 1;
+    o0 = static_cast<int>(s0) * c0;
+    B = kBatch;
+  }
+  // the particle (index within the group, clamped to it) of slot k
+  __device__ inline uint32_t particle(int k, int lane, uint32_t G) const {
+    const uint32_t i = base[k] + static_cast<uint32_t>(lane) % rows[k];
+    return i < G ? i : G - 1;
+  }
   __device__ inline void pp(const V4* ring) {
-    switch (nc) {
-      case 1: eval_pp_group<Real, kMask, 1>(ring, gx, gy, gz, eps2, acc); break;
-      case 2: eval_pp_group<Real, kMask, 2>(ring, gx, gy, gz, eps2, acc); break;
-      case 3: eval_pp_group<Real, kMask, 3>(ring, gx, gy, gz, eps2, acc); break;
-      default: eval_pp_group<Real, kMask, 4>(ring, gx, gy, gz, eps2, acc); break;
+    if (nc == 2) {
+      eval_pp_group<Real, kMask, 2>(ring + o0, c0, gx, gy, gz, eps2, acc);
+    } else {
+      eval_pp_group<Real, kMask, 1>(ring + o0, c0, gx, gy, gz, eps2, acc);
+      if (c1 > 0 && tail_on) eval_pp<Real, kMask>(ring + o1, c1, gx[1], gy[1], gz[1], eps2, acc[1]);
     }
   }
   __device__ inline void pc(const PcRing<Real>& ring) {
-    switch (nc) {
-      case 1: eval_pc_group<Real, 1>(ring, gx, gy, gz, eps2, acc); break;
-      case 2: eval_pc_group<Real, 2>(ring, gx, gy, gz, eps2, acc); break;
-      case 3: eval_pc_group<Real, 3>(ring, gx, gy, gz, eps2, acc); break;
-      default: eval_pc_group<Real, 4>(ring, gx, gy, gz, eps2, acc); break;
+    if (nc == 2) {
+      eval_pc_group<Real, 2>(ring, o0, c0, gx, gy, gz, eps2, acc);
+    } else {
+      eval_pc_group<Real, 1>(ring, o0, c0, gx, gy, gz, eps2, acc);
+      if (c1 > 0 && tail_on) eval_pc<Real>(ring, o1, c1, gx[1], gy[1], gz[1], eps2, acc[1]);
     }
   }
 };
@@ -767,14 +817,11 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   double box[6];
   particle_box(a, gb, G, box);
   GroupEval<Real, kMask> ev;
-  ev.nc = static_cast<int>((G + kWave - 1) / kWave);
-  ev.B = kBatch;
+  ev.setup(G, lane);
   ev.eps2 = static_cast<Real>(sp.eps2);
 #pragma unroll
   for (int k = 0; k < kGroupChunks; ++k) {
-    const uint32_t i = static_cast<uint32_t>(k * kWave + lane);
-    const uint32_t pi = gb + (i < G ? i : G - 1);
-    const double4 q = a.posm[pi];
+    const double4 q = a.posm[gb + ev.particle(k, lane, G)];
     ev.gx[k] = static_cast<Real>(q.x - gcv.x);
     ev.gy[k] = static_cast<Real>(q.y - gcv.y);
     ev.gz[k] = static_cast<Real>(q.z - gcv.z);
@@ -784,13 +831,32 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   const bool ok = traverse<Real, kMask, true>(a, L, gfr, box, gcv.x, gcv.y, gcv.z, sys_id, nullptr, 0,
                                               sp.use_quad != 0, sp.theta2, ev, g_pp, g_pc, dl, n_def);
   if (ok) {
+    // per-particle sums: each slot's slices added in FP64 through LDS (the
+    // P2C ring is free after the walk), then stored for the targets
+    double* red = reinterpret_cast<double*>(L.pc.a);  // 3 x 64 doubles
+    const int used = ev.nc == 2 || ev.c1 > 0 ? 2 : 1;
 #pragma unroll
     for (int k = 0; k < kGroupChunks; ++k) {
-      const uint32_t i = static_cast<uint32_t>(k * kWave + lane);
-      if (k < ev.nc && i < G) {
-        a.gacc[3 * uint64_t(gb + i) + 0] = ev.acc[k].sx();
-        a.gacc[3 * uint64_t(gb + i) + 1] = ev.acc[k].sy();
-        a.gacc[3 * uint64_t(gb + i) + 2] = ev.acc[k].sz();
+      if (k < used) {
+        const uint32_t P = ev.rows[k], S = ev.slices[k];
+        red[3 * lane + 0] = ev.acc[k].sx();
+        red[3 * lane + 1] = ev.acc[k].sy();
+        red[3 * lane + 2] = ev.acc[k].sz();
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t i = ev.base[k] + static_cast<uint32_t>(lane);
+        if (static_cast<uint32_t>(lane) < P && i < G) {
+          double ax = 0.0, ay = 0.0, az = 0.0;
+          for (uint32_t s2 = 0; s2 < S; ++s2) {
+            const uint32_t src = s2 * P + static_cast<uint32_t>(lane);
+            ax += red[3 * src + 0];
+            ay += red[3 * src + 1];
+            az += red[3 * src + 2];
+          }
+          a.gacc[3 * uint64_t(gb + i) + 0] = ax;
+          a.gacc[3 * uint64_t(gb + i) + 1] = ay;
+          a.gacc[3 * uint64_t(gb + i) + 2] = az;
+        }
+        __builtin_amdgcn_wave_barrier();
       }
     }
     // the deferred list and the group sums are read back by other lanes
