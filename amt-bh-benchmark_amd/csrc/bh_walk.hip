@@ -532,11 +532,18 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
     bool accept = false, isleaf = false, defer = false;
     uint32_t lfirst = 0, lcnt = 0;
     double4 mac = make_double4(0, 0, 0, 0);
+    // the record's second half, {first, count, m, Qxx} and {Qyy, Qxy, Qxz,
+    // Qyz}, is loaded with the MAC data: accepted cells are staged from
+    // registers, without a second round trip to memory
+    uint4 rw2 = make_uint4(0, 0, 0, 0), rw3 = make_uint4(0, 0, 0, 0);
     const uint32_t nd = node < last_node ? node : last_node;
     const NodeRec* rp = &a.rec[nd];
     if (nv) {
+      const uint4* rw = reinterpret_cast<const uint4*>(rp);
+      rw2 = rw[2];
+      if constexpr (sizeof(Real) == 4) rw3 = rw[3];
       mac = make_double4(rp->cx, rp->cy, rp->cz, rp->edge2);
-      const uint2 lk = make_uint2(rp->first, rp->count);
+      const uint2 lk = make_uint2(rw2.x, rw2.y);
       const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
       const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
       const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
@@ -600,10 +607,10 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
         if (cellf) {
           const uint32_t slot = pc_cnt + pos;
           if constexpr (sizeof(Real) == 4) {
-            const float qxx = rp->qxx, qyy = rp->qyy;
-            L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, rp->m);
-            L.pc.b[slot] = make_float4(qxx, qyy, -(qxx + qyy), rp->qxy);
-            L.pc.c[slot] = make_float2(rp->qxz, rp->qyz);
+            const float qxx = __uint_as_float(rw2.w), qyy = __uint_as_float(rw3.x);
+            L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, __uint_as_float(rw2.z));
+            L.pc.b[slot] = make_float4(qxx, qyy, -(qxx + qyy), __uint_as_float(rw3.y));
+            L.pc.c[slot] = make_float2(__uint_as_float(rw3.z), __uint_as_float(rw3.w));
           } else {
             const double* q = &a.q64[8 * nd];
             L.pc.a[slot] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, q[0]);
@@ -616,7 +623,7 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
         while (pc_cnt >= B) flush_pc();
       } else {
         if (cellf) {
-          const double m = sizeof(Real) == 4 ? static_cast<double>(rp->m) : a.q64[8 * nd];
+          const double m = sizeof(Real) == 4 ? static_cast<double>(__uint_as_float(rw2.z)) : a.q64[8 * nd];
           L.pp[pp_cnt + pos] = make_entry<Real>(mac.x - cx, mac.y - cy, mac.z - cz, m);
         }
         pp_cnt += nc;
