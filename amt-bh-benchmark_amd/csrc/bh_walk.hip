@@ -1021,7 +1021,10 @@ hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) 
 
 hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s) {
   if (a.n_groups == 0) return hipSuccess;
-  return fp64 ? launch_walk_t<double, 2>(a, any_unsoftened, s) : launch_walk_t<float, 2>(a, any_unsoftened, s);
+  // One wavefront per workgroup: walk groups differ widely in work, and a
+  // workgroup's LDS is released only when all its waves are done (2 waves
+  // per workgroup: 49.7 ms at 16M, 4: 53.9, 8: 64.8; 1: 47.0).
+  return fp64 ? launch_walk_t<double, 1>(a, any_unsoftened, s) : launch_walk_t<float, 1>(a, any_unsoftened, s);
 }
 
 hipError_t launch_potential(const double4* src, uint32_t n, double* partial, uint32_t splits, hipStream_t s) {
