@@ -528,40 +528,37 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
              (static_cast<uint32_t>(lane) - static_cast<uint32_t>(__shfl(static_cast<int>(excl), j)));
     }
     const bool nv = static_cast<uint32_t>(lane) < total;
-    // MAC in FP64, reference operation order (multipole.h:130-160)
-    bool accept = false, isleaf = false, defer = false;
-    uint32_t lfirst = 0, lcnt = 0;
-    double4 mac = make_double4(0, 0, 0, 0);
-    // the record's second half, {first, count, m, Qxx} and {Qyy, Qxy, Qxz,
-    // Qyz}, is loaded with the MAC data: accepted cells are staged from
-    // registers, without a second round trip to memory
-    uint4 rw2 = make_uint4(0, 0, 0, 0), rw3 = make_uint4(0, 0, 0, 0);
+    // MAC in FP64, reference operation order (multipole.h:130-160). Every
+    // lane loads and tests a record (lanes past `total` a clamped, valid
+    // one) and the flags below drop the extra lanes: no divergent branch.
+    // The record's second half, {first, count, m, Qxx} and {Qyy, Qxy, Qxz,
+    // Qyz}, comes with the MAC data, so accepted cells are staged from
+    // registers without a second round trip to memory.
     const uint32_t nd = node < last_node ? node : last_node;
     const NodeRec* rp = &a.rec[nd];
-    if (nv) {
-      const uint4* rw = reinterpret_cast<const uint4*>(rp);
-      rw2 = rw[2];
-      if constexpr (sizeof(Real) == 4) rw3 = rw[3];
-      mac = make_double4(rp->cx, rp->cy, rp->cz, rp->edge2);
-      const uint2 lk = make_uint2(rw2.x, rw2.y);
-      const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
-      const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
-      const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
-      const double near2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
-      accept = theta2 * near2 > mac.w;
-      if constexpr (kGroup) {
-        if (!accept) {
-          const double fx0 = fmax(mac.x - blo0, bhi0 - mac.x);
-          const double fx1 = fmax(mac.y - blo1, bhi1 - mac.y);
-          const double fx2 = fmax(mac.z - blo2, bhi2 - mac.z);
-          const double far2 = fx0 * fx0 + fx1 * fx1 + fx2 * fx2;
-          defer = !(theta2 * far2 <= mac.w);
-        }
+    const uint4* rw = reinterpret_cast<const uint4*>(rp);
+    const uint4 rw2 = rw[2];
+    uint4 rw3 = make_uint4(0, 0, 0, 0);
+    if constexpr (sizeof(Real) == 4) rw3 = rw[3];
+    const double4 mac = make_double4(rp->cx, rp->cy, rp->cz, rp->edge2);
+    bool defer = false;
+    const double dx0 = fmax(fmax(blo0 - mac.x, mac.x - bhi0), 0.0);
+    const double dx1 = fmax(fmax(blo1 - mac.y, mac.y - bhi1), 0.0);
+    const double dx2 = fmax(fmax(blo2 - mac.z, mac.z - bhi2), 0.0);
+    const double near2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+    const bool accept = theta2 * near2 > mac.w;
+    if constexpr (kGroup) {
+      if (!accept) {
+        const double fx0 = fmax(mac.x - blo0, bhi0 - mac.x);
+        const double fx1 = fmax(mac.y - blo1, bhi1 - mac.y);
+        const double fx2 = fmax(mac.z - blo2, bhi2 - mac.z);
+        const double far2 = fx0 * fx0 + fx1 * fx1 + fx2 * fx2;
+        defer = !(theta2 * far2 <= mac.w);
       }
-      isleaf = (lk.y >> 31) != 0;
-      lfirst = lk.x;
-      lcnt = lk.y & 0x7fffffffu;
     }
+    const bool isleaf = (rw2.y >> 31) != 0;
+    const uint32_t lfirst = rw2.x;
+    const uint32_t lcnt = rw2.y & 0x7fffffffu;
     const bool cellf = nv && accept;
     const bool deff = nv && defer;
     const bool leaff = nv && !accept && !defer && isleaf;

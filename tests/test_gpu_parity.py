@@ -158,6 +158,45 @@ def test_large_targets_span_chunks(gpu, orc):
     gpu.set_precision(bl.BH_FP32)
 
 
+def walk_group_sizes(export, tsm, cap=128):
+    """Particle counts of the walk groups that run a shared phase: topmost
+    nodes with leaf || count <= max(tsm, cap) that are not targets."""
+    nodes = export["nodes"]
+    leaf = nodes["leaf"].astype(bool)
+    cnt = nodes["particle_count"].astype(np.int64)
+    fc = nodes["first_child"].astype(np.int64)
+    nch = nodes["num_children"].astype(np.int64)
+    par = np.full(len(cnt), -1)
+    for i in np.nonzero(~leaf)[0]:
+        par[fc[i]:fc[i] + nch[i]] = i
+    pred = leaf | (cnt <= max(tsm, cap))
+    top = pred & ((par < 0) | ~pred[np.maximum(par, 0)])
+    return cnt[top & ~leaf & (cnt > tsm)]
+
+
+@pytest.mark.parametrize("kind,n,ml", [("plummer", 60000, 32), ("hernquist", 40000, 16), ("plummer", 30000, 8)])
+def test_group_layouts_by_size(gpu, orc, kind, n, ml):
+    """The shared phase lays a group's particles over the lanes by its size
+    (two per lane in 3 slices up to 42, one per lane up to 64, a sliced tail
+    up to 96, two per lane up to 128): every range occurs here, and lists and
+    forces match the oracle."""
+    p = getattr(ic, kind)(n, 13)
+    fp = layout.force_params(0.5, 1e-6, 1.0, True)
+    ot = orc.build(p, ml, 256)
+    sizes = walk_group_sizes(ot.export(), 32)
+    for lo, hi in ((33, 42), (43, 64), (65, 96), (97, 128)):
+        assert ((sizes >= lo) & (sizes <= hi)).any(), (lo, hi)
+    oa, opp, opc, otot = ot.grouped_pass(fp, 32)
+    gpu.build_tree(p, ml, 256)
+    for prec in (bl.BH_FP64, bl.BH_FP32):
+        print(f"group layouts {kind} n={n} ml={ml} prec={prec}", flush=True)
+        gpu.set_precision(prec)
+        ga, gpp, gpc, gtot = gpu.accel(fp, 32)
+        assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc) and np.array_equal(gtot, otot)
+        assert rel_err(ga, oa) < TOL[prec]
+    gpu.set_precision(bl.BH_FP32)
+
+
 def test_theta0_equals_direct_sum(gpu):
     p = ic.random_box(1500, 1, 10.0)
     fp = layout.force_params(0.0, 1e-6, 1.0, True)
