@@ -58,6 +58,7 @@ struct Ctx {
   DevBuf<uint32_t> defer;           // deferred nodes per group
   DevBuf<uint32_t> deep_frames;     // walk stack frames of deep trees, per group
   DevBuf<double> gacc;              // group sums per particle
+  DevBuf<double> nbox;              // particle boxes of targets and groups, per node
   // tree (level-order BFS)
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
   DevBuf<uint8_t> depth, leaf, nchild;
@@ -397,6 +398,8 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   // per-target mode: the walk groups are the targets themselves
   HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), c->walk_mode == BH_WALK_GROUPED,
                            c->gmark.p, c->tcounts.p, c->tprefix.p, c->groups.p, c->misc.p + 41, s));
+  TRY(ensure(c, c->nbox, 6ull * c->n_nodes, "node boxes"));
+  HIP_OK(c, launch_node_boxes(c->posm.p, c->begin.p, c->count.p, c->mark.p, c->gmark.p, c->n_nodes, c->nbox.p, s));
   HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
   HIP_OK(c, hipEventRecord(c->ev[3], s));
   HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -416,6 +419,7 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   a.sid = c->sid.p;
   a.sys = sys_dev(c);
   a.gacc = c->gacc.p;
+  a.nbox = c->nbox.p;
   a.defer = c->defer.p;
   a.n_groups = n_groups;
   a.n_nodes = c->n_nodes;
@@ -603,6 +607,7 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->defer);
   release(c->deep_frames);
   release(c->gacc);
+  release(c->nbox);
   release(c->begin);
   release(c->count);
   release(c->cs);

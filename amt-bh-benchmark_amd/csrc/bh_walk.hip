@@ -244,15 +244,6 @@ __device__ inline void eval_pc(const PcRing<Real>& r, int j0, int C, Real tx, Re
   acc.add(bx, by, bz);
 }
 
-__device__ inline double wave_min(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-  return v;
-}
-__device__ inline double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-
 // Group-phase P2P / P2C: every lane holds NC (<= 2) particles of the group
 // and takes the C entries at e (the same entries for both).
 template <typename Real, bool kMask, int NC>
@@ -683,27 +674,6 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
   return true;
 }
 
-// FP64 bounding box of particles [b, b + n) (multipole.h:115-128).
-__device__ inline void particle_box(const WalkArgs& a, uint32_t b, uint32_t n, double* box) {
-  const double inf = __builtin_huge_val();
-  double lo0 = inf, lo1 = inf, lo2 = inf, hi0 = -inf, hi1 = -inf, hi2 = -inf;
-  for (uint32_t p = __lane_id(); p < n; p += kWave) {
-    const double4 q = a.posm[b + p];
-    lo0 = fmin(lo0, q.x);
-    lo1 = fmin(lo1, q.y);
-    lo2 = fmin(lo2, q.z);
-    hi0 = fmax(hi0, q.x);
-    hi1 = fmax(hi1, q.y);
-    hi2 = fmax(hi2, q.z);
-  }
-  box[0] = wave_min(lo0);
-  box[1] = wave_min(lo1);
-  box[2] = wave_min(lo2);
-  box[3] = wave_max(hi0);
-  box[4] = wave_max(hi1);
-  box[5] = wave_max(hi2);
-}
-
 // One target (node tnode): the walk over `start` (the group's deferred
 // nodes) or from the root, for each chunk of <= 64 of its particles; then
 // the slices' sums, plus the group's share (gacc, counts g_pp / g_pc) when
@@ -720,8 +690,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
   const bool use_quad = sp.use_quad != 0;
   const double4 tcv = a.ctr[tnode];
   const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
-  double box[6];
-  particle_box(a, tb, T, box);
+  const double* box = a.nbox + 6 * uint64_t(tnode);
   const double theta2 = sp.theta2;
 
   for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
@@ -818,8 +787,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
     return;
   }
   const double4 gcv = a.ctr[g];
-  double box[6];
-  particle_box(a, gb, G, box);
+  const double* box = a.nbox + 6 * uint64_t(g);
   GroupEval<Real, kMask> ev;
   ev.setup(G, lane);
   ev.eps2 = static_cast<Real>(sp.eps2);
