@@ -58,7 +58,7 @@ struct Ctx {
   DevBuf<uint32_t> defer;           // deferred nodes per group
   DevBuf<uint32_t> deep_frames;     // walk stack frames of deep trees, per group
   DevBuf<double> gacc;              // group sums per particle
-  DevBuf<double> nbox;              // particle boxes of targets and groups, per node
+  DevBuf<double> nbox;              // particle box per node
   // tree (level-order BFS)
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
   DevBuf<uint8_t> depth, leaf, nchild;
@@ -187,6 +187,7 @@ TreeDev tree_view(Ctx* c) {
   t.iab = c->iab.p;
   t.rec = c->rec.p;
   t.q64 = c->q64.p;
+  t.nbox = c->nbox.p;
   t.leaf_node = c->leaf_node.p;
   t.leaf_first = c->leaf_first.p;
   t.level_off = c->level_off.p;
@@ -364,6 +365,7 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n) {
   TRY(ensure(c, c->com, 3ull * m, "nodes"));
   TRY(ensure(c, c->iab, 6ull * m, "nodes"));
   TRY(ensure(c, c->q64, 8ull * m, "nodes"));
+  TRY(ensure(c, c->nbox, 6ull * m, "nodes"));
 
   TRY(ensure(c, c->leaf_node, leaves, "leaves"));
   TRY(ensure(c, c->leaf_first, leaves, "leaves"));
@@ -398,8 +400,6 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   // per-target mode: the walk groups are the targets themselves
   HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), c->walk_mode == BH_WALK_GROUPED,
                            c->gmark.p, c->tcounts.p, c->tprefix.p, c->groups.p, c->misc.p + 41, s));
-  TRY(ensure(c, c->nbox, 6ull * c->n_nodes, "node boxes"));
-  HIP_OK(c, launch_node_boxes(c->posm.p, c->begin.p, c->count.p, c->mark.p, c->gmark.p, c->n_nodes, c->nbox.p, s));
   HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
   HIP_OK(c, hipEventRecord(c->ev[3], s));
   HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -77,6 +77,7 @@ struct TreeDev {
   double* iab;           // [6M] xx,yy,zz,xy,xz,yz about the com
   NodeRec* rec;          // [M] the walk's node record
   double* q64;           // [8M] mass, Qxx, Qyy, Qzz, Qxy, Qxz, Qyz, 0
+  double* nbox;          // [6M] FP64 box of the node's particles: lo x, y, z, hi x, y, z
   uint32_t* leaf_node;   // [leaves] node of each leaf, Morton order
   uint32_t* leaf_first;  // [leaves] first particle of each leaf
   uint32_t* level_off;   // [kLevels + 2] level start offsets, then total, total
@@ -93,7 +94,7 @@ struct WalkArgs {
   const uint32_t* tmark;   // [n] target node + 1 at each target's first particle, else 0
   const uint16_t* sid;     // system of each particle
   const SysDev* sys;       // system table; the root of system s is node s
-  const double* nbox;      // [6 n_nodes] FP64 particle box (lo xyz, hi xyz) of every target and walk group
+  const double* nbox;      // [6 n_nodes] FP64 particle box of every node (lo xyz, hi xyz)
   double* gacc;            // [3n] scratch: a group's shared-list sums per particle
   uint32_t* defer;         // [n_groups * kDefer] scratch: deferred nodes per group
   uint32_t n_groups;
@@ -143,12 +144,6 @@ hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_
 hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
                           uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
                           uint32_t* n_targets, hipStream_t s);
-
-// FP64 bounding box of the particles of every node marked in tmark or
-// gmark (the walk's targets and groups; multipole.h:115-128), into
-// nbox[6 * node] = {lo x, lo y, lo z, hi x, hi y, hi z}.
-hipError_t launch_node_boxes(const double4* posm, const uint32_t* begin, const uint32_t* count, const uint32_t* tmark,
-                             const uint32_t* gmark, uint32_t n_nodes, double* nbox, hipStream_t s);
 
 // ---- bh_walk.hip
 hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s);

@@ -150,14 +150,14 @@ extern "C" {
 // (128 B), sort keys and indices (24), the walk copy (32), build and target
 // scratch (~20), the group sums (24) and marks (8), and per walk group (at
 // least one per 128 particles, ~40 particles on average) its deferred list
-// (2 KB) and deep stack frames (2.5 KB): ~350 B; per tree node ~280 B, with
+// (2 KB) and deep stack frames (2.5 KB): ~350 B; per tree node ~330 B, with
 // nodes ~ n min(1.6, 4 / max_leaf) (measured on Plummer, Hernquist and
 // uniform trees).
 uint64_t bh_server_footprint(const bh_request* r) {
   if (!r) return 0;
   const double per_node = 4.0 / std::max(1u, r->max_leaf);
   const double nodes = static_cast<double>(r->n) * std::min(1.6, per_node) + 64.0;
-  return static_cast<uint64_t>(static_cast<double>(r->n) * 350.0 + nodes * 280.0);
+  return static_cast<uint64_t>(static_cast<double>(r->n) * 350.0 + nodes * 330.0);
 }
 
 int bh_server_create(bh_server** out, bh_ctx* ctx, uint64_t mem_cap_bytes, uint32_t max_systems) {

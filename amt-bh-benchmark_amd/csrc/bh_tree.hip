@@ -282,6 +282,10 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
     const double4 cc = t.ctr[i];
     const double ref[3] = {cc.x, cc.y, cc.z};
     double rm = 0.0, m1[3] = {0.0, 0.0, 0.0}, m2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // the FP64 box of the node's particles (multipole.h:115-128; exact, in
+    // any order): a leaf's from its particles, a parent's from its children
+    const double inf = __builtin_huge_val();
+    double blo[3] = {inf, inf, inf}, bhi[3] = {-inf, -inf, -inf};
     uint32_t cnt;
     const bool leaf = t.leaf[i] != 0;
     if (leaf) {
@@ -290,6 +294,12 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
       for (uint32_t p = b; p < b + cnt; ++p) {
         const double4 q = posm[p];
         const double mass = q.w;
+        blo[0] = fmin(blo[0], q.x);
+        blo[1] = fmin(blo[1], q.y);
+        blo[2] = fmin(blo[2], q.z);
+        bhi[0] = fmax(bhi[0], q.x);
+        bhi[1] = fmax(bhi[1], q.y);
+        bhi[2] = fmax(bhi[2], q.z);
         const double d[3] = {q.x - ref[0], q.y - ref[1], q.z - ref[2]};
         rm += mass;
         for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
@@ -316,6 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
         m2[4] += ib[4] + mass * d[0] * d[2];
         m2[5] += ib[5] + mass * d[1] * d[2];
         cnt += t.count[c];
+        const double* cb = &t.nbox[6 * uint64_t(c)];
+        for (int a = 0; a < 3; ++a) {
+          blo[a] = fmin(blo[a], cb[a]);
+          bhi[a] = fmax(bhi[a], cb[3 + a]);
+        }
       }
       t.count[i] = cnt;
     }
@@ -330,6 +345,10 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
     ib[3] = m2[3] - rm * dl[0] * dl[1];
     ib[4] = m2[4] - rm * dl[0] * dl[2];
     ib[5] = m2[5] - rm * dl[1] * dl[2];
+    for (int a = 0; a < 3; ++a) {
+      t.nbox[6 * uint64_t(i) + a] = blo[a];
+      t.nbox[6 * uint64_t(i) + 3 + a] = bhi[a];
+    }
     t.mass[i] = rm;
     for (int a = 0; a < 3; ++a) t.com[3 * i + a] = com[a];
     for (int a = 0; a < 6; ++a) t.iab[6 * i + a] = ib[a];
@@ -416,37 +435,6 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const uint32_t* __rest
   }
 }
 
-// One thread per node; the walk's targets and groups (marked at their first
-// particle) get their particles' FP64 box. Exact: min and max do not round.
-__global__ __launch_bounds__(kBlock) void k_node_boxes(const double4* __restrict__ posm, const uint32_t* __restrict__ begin,
-                                                       const uint32_t* __restrict__ count, const uint32_t* __restrict__ tmark,
-                                                       const uint32_t* __restrict__ gmark, uint32_t n_nodes,
-                                                       double* __restrict__ nbox) {
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
-    const uint32_t b = begin[i];
-    if (tmark[b] != i + 1 && gmark[b] != i + 1) continue;
-    const uint32_t c = count[i];
-    const double inf = __builtin_huge_val();
-    double lo0 = inf, lo1 = inf, lo2 = inf, hi0 = -inf, hi1 = -inf, hi2 = -inf;
-    for (uint32_t k = 0; k < c; ++k) {
-      const double4 q = posm[b + k];
-      lo0 = fmin(lo0, q.x);
-      lo1 = fmin(lo1, q.y);
-      lo2 = fmin(lo2, q.z);
-      hi0 = fmax(hi0, q.x);
-      hi1 = fmax(hi1, q.y);
-      hi2 = fmax(hi2, q.z);
-    }
-    double* o = nbox + 6 * uint64_t(i);
-    o[0] = lo0;
-    o[1] = lo1;
-    o[2] = lo2;
-    o[3] = hi0;
-    o[4] = hi1;
-    o[5] = hi2;
-  }
-}
-
 inline uint32_t grid_for(uint64_t n, uint32_t cap) {
   uint64_t g = (n + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
@@ -498,13 +486,6 @@ hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_
   hipLaunchKernelGGL(k_compact_count, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_counts);
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, 1u, tile_prefix, n_targets);
   hipLaunchKernelGGL(k_compact_write, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_prefix, targets);
-  return hipGetLastError();
-}
-
-hipError_t launch_node_boxes(const double4* posm, const uint32_t* begin, const uint32_t* count, const uint32_t* tmark,
-                             const uint32_t* gmark, uint32_t n_nodes, double* nbox, hipStream_t s) {
-  hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n_nodes, 8192)), dim3(kBlock), 0, s, posm, begin, count, tmark, gmark,
-                     n_nodes, nbox);
   return hipGetLastError();
 }
 
