@@ -59,6 +59,7 @@ struct Ctx {
   DevBuf<uint32_t> deep_frames;     // walk stack frames of deep trees, per group
   DevBuf<double> gacc;              // group sums per particle
   DevBuf<double> nbox;              // particle box per node
+  DevBuf<uint32_t> unit_n;          // walk units packed from single-target groups, per node
   // tree (level-order BFS)
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
   DevBuf<uint8_t> depth, leaf, nchild;
@@ -396,10 +397,12 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   TRY(ensure(c, c->gacc, 3ull * n, "group sums"));
   TRY(ensure(c, c->totals, 2, "totals"));
   HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), false, c->mark.p, c->tcounts.p,
-                           c->tprefix.p, c->targets.p, c->misc.p + 40, s));
+                           c->tprefix.p, c->targets.p, c->misc.p + 40, nullptr, s));
   // per-target mode: the walk groups are the targets themselves
-  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), c->walk_mode == BH_WALK_GROUPED,
-                           c->gmark.p, c->tcounts.p, c->tprefix.p, c->groups.p, c->misc.p + 41, s));
+  const bool grouped = c->walk_mode == BH_WALK_GROUPED;
+  if (grouped) TRY(ensure(c, c->unit_n, c->n_nodes, "walk units"));
+  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), grouped, c->gmark.p, c->tcounts.p,
+                           c->tprefix.p, c->groups.p, c->misc.p + 41, grouped ? c->unit_n.p : nullptr, s));
   HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
   HIP_OK(c, hipEventRecord(c->ev[3], s));
   HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -415,6 +418,7 @@ int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d
   a.count = c->count.p;
   a.posm = c->posm.p;
   a.groups = c->groups.p;
+  a.unit_n = grouped ? c->unit_n.p : nullptr;
   a.tmark = c->mark.p;
   a.sid = c->sid.p;
   a.sys = sys_dev(c);
@@ -608,6 +612,7 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->deep_frames);
   release(c->gacc);
   release(c->nbox);
+  release(c->unit_n);
   release(c->begin);
   release(c->count);
   release(c->cs);

@@ -90,7 +90,8 @@ struct WalkArgs {
   const uint32_t* begin;
   const uint32_t* count;
   const double4* posm;     // x, y, z, m in Morton order
-  const uint32_t* groups;  // walk groups (nodes), Morton order
+  const uint32_t* groups;  // walk units (first node of each), Morton order
+  const uint32_t* unit_n;  // [n_nodes] particles | 1 << 31 of a packed unit at its first node, else 0 (nullable)
   const uint32_t* tmark;   // [n] target node + 1 at each target's first particle, else 0
   const uint16_t* sid;     // system of each particle
   const SysDev* sys;       // system table; the root of system s is node s
@@ -140,10 +141,12 @@ hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_
                               const double* lohi, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s);
 // Marks (node + 1 at its first particle) and lists the topmost nodes with
 // leaf || count <= thr: thr = the system's tsm (walk targets) or, with
-// groups set, max(tsm, kGroupCap) (walk groups).
+// groups set, max(tsm, kGroupCap) (walk groups). With groups and unit_n,
+// runs of sibling single-target groups are packed into walk units (unit_n
+// per node: particles | 1 << 31 at a unit's first node, else 0).
 hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
                           uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
-                          uint32_t* n_targets, hipStream_t s);
+                          uint32_t* n_targets, uint32_t* unit_n, hipStream_t s);
 
 // ---- bh_walk.hip
 hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s);

@@ -392,6 +392,45 @@ __global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_no
   }
 }
 
+// Walk units from the walk groups. Below a node with more than
+// max(tsm, kGroupCap) particles, a run of consecutive children that are
+// single targets (leaf || count <= tsm) is packed greedily into units of at
+// most kGroupCap particles; a unit of two or more walks as one group (one
+// shared walk, then its targets), the others keep their own walks. The
+// unit's first child keeps its mark and gets unit_n = particles | 1 << 31,
+// the other members lose theirs; every other walk group gets unit_n = 0.
+// Any box holding the targets' boxes gives the same lists (near2 and far2
+// are monotone in the box), so a unit is exact whatever it groups.
+__global__ __launch_bounds__(kBlock) void k_group_units(TreeDev t, uint32_t n_nodes, const uint16_t* __restrict__ sid,
+                                                        const SysDev* __restrict__ sys, uint32_t* __restrict__ mark,
+                                                        uint32_t* __restrict__ unit_n) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
+    const uint32_t tsm = sys[sid[t.begin[i]]].tsm;
+    const uint32_t thr = max(tsm, kGroupCap);
+    const bool is_group = t.leaf[i] || t.count[i] <= thr;
+    if (t.depth[i] == 0 && is_group) unit_n[i] = 0u;  // a system that is one walk group
+    if (is_group) continue;
+    const uint32_t fc = t.first_child[i], k = t.nchild[i];
+    uint32_t run = 0, run_len = 0, run_n = 0;
+    for (uint32_t c = fc; c <= fc + k; ++c) {
+      const bool in = c < fc + k;
+      const uint32_t cn = in ? t.count[c] : 0u;
+      const bool single = in && (t.leaf[c] || cn <= tsm) && cn <= kGroupCap;
+      if (in && (t.leaf[c] || cn <= thr)) unit_n[c] = 0u;
+      if (single && run_len > 0 && run_n + cn <= kGroupCap) {
+        mark[t.begin[c]] = 0u;
+        ++run_len;
+        run_n += cn;
+        continue;
+      }
+      if (run_len > 1) unit_n[run] = run_n | 0x80000000u;
+      run_len = single ? 1u : 0u;
+      run = c;
+      run_n = cn;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_compact_count(const uint32_t* __restrict__ mark, uint32_t n,
                                                           uint32_t* __restrict__ tile_counts) {
   __shared__ uint32_t s;
@@ -477,12 +516,15 @@ hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_
 
 hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
                           uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
-                          uint32_t* n_targets, hipStream_t s) {
+                          uint32_t* n_targets, uint32_t* unit_n, hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   hipError_t e = launch_fill_u32(mark, 0u, n, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys,
                      groups ? kGroupCap : 0u, mark);
+  if (groups && unit_n)
+    hipLaunchKernelGGL(k_group_units, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys, mark,
+                       unit_n);
   hipLaunchKernelGGL(k_compact_count, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_counts);
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, 1u, tile_prefix, n_targets);
   hipLaunchKernelGGL(k_compact_write, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_prefix, targets);

@@ -12,15 +12,17 @@
 //   P2C               multipole.h:171-200: monopole + traceless quadrupole
 //   kick/drift        integrate.h:15-16 via integrate_ttg.h:119-122, fused
 //
-// Work is split into walk groups: topmost nodes with leaf || count <=
-// max(tsm, 256), one wavefront each. A group that is itself a target walks
-// as one target. Otherwise the wavefront first walks the tree for the
-// group's particle box with the reference's three-way decision at a group
-// node (grouped_ttg.h:259-276): accepted (theta^2 near2 > edge^2) -> the
-// shared list, rejected for every box inside (theta^2 far2 <= edge^2) ->
-// opened for all, otherwise deferred. The shared list is evaluated for all
-// the group's particles at once (each lane holds up to 4 of them). Then
-// each target of the group walks only the deferred nodes. This gives every
+// Work is split into walk units, one wavefront each: walk groups (topmost
+// nodes with leaf || count <= max(tsm, 128)), with runs of sibling groups
+// that are single targets packed into units of up to 128 particles. A
+// group that is itself a target (and not packed) walks as one target.
+// Otherwise the wavefront first walks the tree for the unit's particle box
+// with the reference's three-way decision at a group node
+// (grouped_ttg.h:259-276): accepted (theta^2 near2 > edge^2) -> the shared
+// list, rejected for every box inside (theta^2 far2 <= edge^2) -> opened
+// for all, otherwise deferred. The shared list is evaluated for all the
+// unit's particles at once. Then each target of the unit walks only the
+// deferred nodes. This gives every
 // target exactly the interaction list of its own walk from the root: near2
 // and far2 are monotone in the box (also in floating point) and near2 <=
 // far2, so a group's accept / reject-for-all is the decision of every
@@ -174,6 +176,11 @@ __device__ inline double uniform_d(double v) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+// A walk box (FP64): lo x, y, z, hi x, y, z.
+struct Box6 {
+  double v[6];
+};
+
 template <typename Real>
 __device__ inline typename Vec4T<Real>::T make_entry(double x, double y, double z, double m) {
   using V4 = typename Vec4T<Real>::T;
@@ -313,7 +320,8 @@ struct TargetEval {
 // entries). Each lane has two particle slots; slot k of lane l holds
 // particle base_k + l % P_k and takes slice l / P_k of S_k slices (C_k =
 // kBatch / S_k consecutive entries of each batch). Layouts, by G:
-//   G <= 42      two slots, P = ceil(G / 2), S = 3 (one fused loop)
+//   G <= 42      two slots, P = ceil(G / 2), S = the largest divisor of
+//                kBatch with P S <= 64 (one fused loop)
 //   43 .. 64     one slot, P = 64, S = 1
 //   65 .. 96     slot 0: particles 0..63, every entry; slot 1: the r = G - 64
 //                others, S_1 = the largest divisor of kBatch with r S_1 <= 64
@@ -336,13 +344,18 @@ struct GroupEval {
     c1 = 0;
     o1 = 0;
     tail_on = false;
+    constexpr uint32_t kDiv[9] = {48u, 24u, 16u, 12u, 8u, 6u, 4u, 3u, 2u};  // divisors of kBatch
     if (G <= 42) {
       const uint32_t P = (G + 1) / 2;
+      uint32_t S = 1;
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        if (S == 1 && kDiv[q] * P <= static_cast<uint32_t>(kWave)) S = kDiv[q];
       nc = 2;
       base[0] = 0;
       base[1] = P;
       rows[0] = rows[1] = P;
-      slices[0] = slices[1] = 3;
+      slices[0] = slices[1] = S;
     } else if (G <= 64) {
       nc = 1;
       base[0] = base[1] = 0;
@@ -350,7 +363,6 @@ struct GroupEval {
       slices[0] = slices[1] = 1;
     } else if (G <= 96) {
       const uint32_t r = G - kWave;
-      constexpr uint32_t kDiv[9] = {48u, 24u, 16u, 12u, 8u, 6u, 4u, 3u, 2u};  // divisors of kBatch
       uint32_t S1 = 1;
 #pragma unroll
       for (int q = 0; q < 9; ++q)
@@ -412,7 +424,7 @@ struct GroupEval {
 // (theta^2 * far2 > edge^2) is deferred: appended to dl, for its targets.
 // Returns false when more than kDefer nodes would be deferred.
 template <typename Real, bool kMask, bool kGroup, typename Eval, typename Lds>
-__device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double* box, double cx, double cy, double cz,
+__device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& box, double cx, double cy, double cz,
                          uint32_t sys_id, const uint32_t* start, uint32_t n_start, bool use_quad, double theta2,
                          Eval& ev, uint32_t& n_pp, uint32_t& n_pc, uint32_t* dl, uint32_t& n_def) {
   constexpr int F = Lds::kFrames;
@@ -422,8 +434,8 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
   const uint64_t lt = lanemask_lt();
   const uint32_t last_node = a.n_nodes - 1, last_part = a.n_parts - 1;
   const uint32_t B = ev.B;
-  const double blo0 = uniform_d(box[0]), blo1 = uniform_d(box[1]), blo2 = uniform_d(box[2]);
-  const double bhi0 = uniform_d(box[3]), bhi1 = uniform_d(box[4]), bhi2 = uniform_d(box[5]);
+  const double blo0 = uniform_d(box.v[0]), blo1 = uniform_d(box.v[1]), blo2 = uniform_d(box.v[2]);
+  const double bhi0 = uniform_d(box.v[3]), bhi1 = uniform_d(box.v[4]), bhi2 = uniform_d(box.v[5]);
   cx = uniform_d(cx);
   cy = uniform_d(cy);
   cz = uniform_d(cz);
@@ -674,6 +686,47 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const double*
   return true;
 }
 
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// FP64 bounding box of particles [b, b + n) (multipole.h:115-128).
+__device__ inline Box6 particle_box(const WalkArgs& a, uint32_t b, uint32_t n) {
+  const double inf = __builtin_huge_val();
+  double lo0 = inf, lo1 = inf, lo2 = inf, hi0 = -inf, hi1 = -inf, hi2 = -inf;
+  for (uint32_t p = __lane_id(); p < n; p += kWave) {
+    const double4 q = a.posm[b + p];
+    lo0 = fmin(lo0, q.x);
+    lo1 = fmin(lo1, q.y);
+    lo2 = fmin(lo2, q.z);
+    hi0 = fmax(hi0, q.x);
+    hi1 = fmax(hi1, q.y);
+    hi2 = fmax(hi2, q.z);
+  }
+  Box6 box;
+  box.v[0] = uniform_d(wave_min(lo0));
+  box.v[1] = uniform_d(wave_min(lo1));
+  box.v[2] = uniform_d(wave_min(lo2));
+  box.v[3] = uniform_d(wave_max(hi0));
+  box.v[4] = uniform_d(wave_max(hi1));
+  box.v[5] = uniform_d(wave_max(hi2));
+  return box;
+}
+
+// A node's particle box as computed by the upward pass.
+__device__ inline Box6 node_box(const WalkArgs& a, uint32_t node) {
+  const double* p = a.nbox + 6 * uint64_t(node);
+  Box6 box;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) box.v[k] = p[k];
+  return box;
+}
+
 // One target (node tnode): the walk over `start` (the group's deferred
 // nodes) or from the root, for each chunk of <= 64 of its particles; then
 // the slices' sums, plus the group's share (gacc, counts g_pp / g_pc) when
@@ -690,7 +743,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
   const bool use_quad = sp.use_quad != 0;
   const double4 tcv = a.ctr[tnode];
   const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
-  const double* box = a.nbox + 6 * uint64_t(tnode);
+  const Box6 box = node_box(a, tnode);
   const double theta2 = sp.theta2;
 
   for (uint32_t c0 = 0; c0 < T; c0 += kWave) {
@@ -768,26 +821,35 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
   }
 }
 
-// One walk group g (topmost node with leaf || count <= max(tsm, 256)). A
-// group that is itself a target walks as one. Otherwise the wave walks the
-// tree once for the group's box - the decisions every target inside shares
-// - evaluating the shared list for all the group's particles and keeping
-// the deferred nodes; then each target of the group walks only those.
+// One walk unit starting at node g: a walk group (topmost node with leaf ||
+// count <= max(tsm, 128)) or a run of sibling single-target groups packed
+// into one unit (unit_n). A group that is itself a target walks as one.
+// Otherwise the wave walks the tree once for the unit's box - the decisions
+// every target inside shares - evaluating the shared list for all the
+// unit's particles and keeping the deferred nodes; then each target of the
+// unit walks only those.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g) {
   const int lane = __lane_id();
   const uint32_t gb = uniform(a.begin[g]);
-  const uint32_t G = uniform(a.count[g]);
+  // a unit packed from sibling single-target groups spans several nodes
+  const uint32_t un = a.unit_n ? uniform(a.unit_n[g]) : 0u;
+  const bool packed = (un >> 31) != 0;
+  const uint32_t G = packed ? un & 0x7fffffffu : uniform(a.count[g]);
   const uint32_t sys_id = uniform(a.sid[gb]);
   const SysDev& sp = a.sys[sys_id];
   const bool gleaf = (a.rec[g].count >> 31) != 0;
   uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(unit) * (kLevels - Lds::kFrames) * kWave : nullptr;
-  if (gleaf || G <= sp.tsm || G > kGroupCap) {
+  if (!packed && (gleaf || G <= sp.tsm || G > kGroupCap)) {
     walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0);
     return;
   }
-  const double4 gcv = a.ctr[g];
-  const double* box = a.nbox + 6 * uint64_t(g);
+  // a packed unit's box and the origin of its relative coordinates (the
+  // box centre); a group's: its node box and centre
+  const Box6 box = packed ? particle_box(a, gb, G) : node_box(a, g);
+  const double4 gcv = packed ? make_double4(0.5 * (box.v[0] + box.v[3]), 0.5 * (box.v[1] + box.v[4]),
+                                            0.5 * (box.v[2] + box.v[5]), 0.0)
+                             : a.ctr[g];
   GroupEval<Real, kMask> ev;
   ev.setup(G, lane);
   ev.eps2 = static_cast<Real>(sp.eps2);

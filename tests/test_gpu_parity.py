@@ -158,6 +158,28 @@ def test_large_targets_span_chunks(gpu, orc):
     gpu.set_precision(bl.BH_FP32)
 
 
+def packed_runs(export, tsm, cap=128):
+    """Number of walk units the walk packs from runs (two or more) of
+    consecutive sibling groups that are single targets, under nodes with
+    more than max(tsm, cap) particles, greedily up to cap particles."""
+    nodes = export["nodes"]
+    leaf = nodes["leaf"].astype(bool)
+    cnt = nodes["particle_count"].astype(np.int64)
+    fc = nodes["first_child"].astype(np.int64)
+    nch = nodes["num_children"].astype(np.int64)
+    units = 0
+    for i in np.nonzero(~leaf & (cnt > max(tsm, cap)))[0]:
+        run_len, run_n = 0, 0
+        for c in list(range(fc[i], fc[i] + nch[i])) + [None]:
+            single = c is not None and (leaf[c] or cnt[c] <= tsm) and cnt[c] <= cap
+            if single and run_len > 0 and run_n + cnt[c] <= cap:
+                run_len, run_n = run_len + 1, run_n + cnt[c]
+                continue
+            units += run_len > 1
+            run_len, run_n = (1, cnt[c]) if single else (0, 0)
+    return units
+
+
 def walk_group_sizes(export, tsm, cap=128):
     """Particle counts of the walk groups that run a shared phase: topmost
     nodes with leaf || count <= max(tsm, cap) that are not targets."""
@@ -176,16 +198,19 @@ def walk_group_sizes(export, tsm, cap=128):
 
 @pytest.mark.parametrize("kind,n,ml", [("plummer", 60000, 32), ("hernquist", 40000, 16), ("plummer", 30000, 8)])
 def test_group_layouts_by_size(gpu, orc, kind, n, ml):
-    """The shared phase lays a group's particles over the lanes by its size
-    (two per lane in 3 slices up to 42, one per lane up to 64, a sliced tail
-    up to 96, two per lane up to 128): every range occurs here, and lists and
-    forces match the oracle."""
+    """The shared phase lays a unit's particles over the lanes by its size
+    (two per lane in slices up to 42, one per lane up to 64, a sliced tail
+    up to 96, two per lane up to 128), and runs of sibling single-target
+    groups are packed into units: every range and packed units occur here,
+    and lists and forces match the oracle."""
     p = getattr(ic, kind)(n, 13)
     fp = layout.force_params(0.5, 1e-6, 1.0, True)
     ot = orc.build(p, ml, 256)
-    sizes = walk_group_sizes(ot.export(), 32)
+    ex = ot.export()
+    sizes = walk_group_sizes(ex, 32)
     for lo, hi in ((33, 42), (43, 64), (65, 96), (97, 128)):
         assert ((sizes >= lo) & (sizes <= hi)).any(), (lo, hi)
+    assert packed_runs(ex, 32) > 0
     oa, opp, opc, otot = ot.grouped_pass(fp, 32)
     gpu.build_tree(p, ml, 256)
     for prec in (bl.BH_FP64, bl.BH_FP32):
