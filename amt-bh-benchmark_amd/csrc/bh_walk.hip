@@ -532,12 +532,14 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
     }
     const bool nv = static_cast<uint32_t>(lane) < total;
     // MAC in FP64, reference operation order (multipole.h:130-160). Every
-    // lane loads and tests a record (lanes past `total` a clamped, valid
-    // one) and the flags below drop the extra lanes: no divergent branch.
+    // lane loads and tests a record (lanes past `total` lane 0's, a line
+    // already being fetched) and the flags below drop the extra lanes: no
+    // divergent branch.
     // The record's second half, {first, count, m, Qxx} and {Qyy, Qxy, Qxz,
     // Qyz}, comes with the MAC data, so accepted cells are staged from
     // registers without a second round trip to memory.
-    const uint32_t nd = node < last_node ? node : last_node;
+    const uint32_t node0 = uniform(node);
+    const uint32_t nd = nv ? (node < last_node ? node : last_node) : (node0 < last_node ? node0 : last_node);
     const NodeRec* rp = &a.rec[nd];
     const uint4* rw = reinterpret_cast<const uint4*>(rp);
     const uint4 rw2 = rw[2];
