@@ -372,7 +372,9 @@ int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n) {
   TRY(ensure(c, c->leaf_first, leaves, "leaves"));
   const TreeDev t = tree_view(c);
   HIP_OK(c, launch_tree_emit(c->L.p, c->H.p, n, c->tprefix.p, c->misc.p + 8, t, s));
-  HIP_OK(c, launch_tree_finish(keys_sorted, c->posm.p, n, c->sid.p, c->max_depth, c->lohi.p, m, leaves, t, s));
+  uint32_t level_nodes[kLevels];
+  for (int d = 0; d < kLevels; ++d) level_nodes[d] = c->host_misc[8 + d];
+  HIP_OK(c, launch_tree_finish(keys_sorted, c->posm.p, n, c->sid.p, c->max_depth, c->lohi.p, m, leaves, level_nodes, t, s));
   HIP_OK(c, hipEventRecord(c->ev[2], s));
   c->n = n;
   c->n_nodes = m;

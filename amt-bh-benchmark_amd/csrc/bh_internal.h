@@ -137,8 +137,10 @@ hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_
                             hipStream_t s);
 hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix,
                             const uint32_t* totals, TreeDev t, hipStream_t s);
+// level_nodes: host array, nodes per level (sizes the upward launches)
 hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t max_depth,
-                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s);
+                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, const uint32_t* level_nodes, TreeDev t,
+                              hipStream_t s);
 // Marks (node + 1 at its first particle) and lists the topmost nodes with
 // leaf || count <= thr: thr = the system's tsm (walk targets) or, with
 // groups set, max(tsm, kGroupCap) (walk groups). With groups and unit_n,

@@ -504,12 +504,14 @@ hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, cons
 }
 
 hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t max_depth,
-                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, TreeDev t, hipStream_t s) {
+                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, const uint32_t* level_nodes, TreeDev t,
+                              hipStream_t s) {
   hipLaunchKernelGGL(k_tree_links, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, keys, n, sid, lohi, n_nodes, n_leaves,
                      t);
-  // bottom-up, one launch per non-empty level; each reads its level range on the device
+  // bottom-up, one launch per non-empty level, sized by the level's node
+  // count; each reads its level range on the device
   for (int d = static_cast<int>(max_depth < kDepthCap ? max_depth : kDepthCap); d >= 0; --d) {
-    hipLaunchKernelGGL(k_upward, dim3(grid_for(n_nodes, 2048)), dim3(kBlock), 0, s, posm, d, t);
+    hipLaunchKernelGGL(k_upward, dim3(grid_for(level_nodes[d], 2048)), dim3(kBlock), 0, s, posm, d, t);
   }
   return hipGetLastError();
 }
