@@ -64,3 +64,30 @@ def test_plummer_16m_properties(gpu):
     assert q["mass"].sum() == pytest.approx(1.0, rel=1e-12)
     st = gpu.last_step_stats()
     assert st.n_pp > 0 and st.n_pc > 0 and st.walk_ms > 0
+
+
+def test_plummer_65k_fp64_leapfrog(gpu, orc):
+    """The CPU-runnable configuration in full: Plummer N = 65,536 (seed 1),
+    eps = 0.01, theta = 0.5, leaf 16, ten leapfrog steps of dt = 1e-3
+    (main.cc:199-246, eleven force evaluations), FP64 walk, against the
+    oracle step by step; then the final forces against the FP64 direct sum."""
+    p = ic.plummer(1 << 16, 1)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    dt, steps = 1e-3, 10
+    gpu.set_precision(bl.BH_FP64)
+    try:
+        gpu.state_upload(p)
+        q = p
+        for k, d in [(dt / 2, dt)] + [(dt, dt)] * (steps - 1) + [(dt / 2, 0.0)]:
+            print(f"plummer 65k fp64: step kick={k} drift={d}", flush=True)
+            gpu.state_step(fp, 16, 256, 32, k, d)
+            q, _ = orc.step(q, fp, 16, 256, 32, k, d)
+        out = gpu.state_download()
+        assert np.array_equal(out["id"], q["id"])
+        assert np.abs(out["pos"] - q["pos"]).max() < 1e-10
+        assert np.abs(out["vel"] - q["vel"]).max() < 1e-9
+        gpu.build_tree(out, 16, 256)
+        rms, p99 = subset_errors(gpu, out, fp, 32)
+        assert rms < 1e-3 and p99 < 5e-3, (rms, p99)
+    finally:
+        gpu.set_precision(bl.BH_FP32)
