@@ -198,12 +198,12 @@ __device__ inline typename Vec4T<Real>::T pad_entry() {
   return make_entry<Real>(1e18, 1e18, 1e18, 0.0);
 }
 
-// P2P over this lane's C consecutive entries (grouped_ttg.h:137-153).
+// P2P over this lane's C consecutive entries (grouped_ttg.h:137-153),
+// added to the plain partial sums bx, by, bz.
 template <typename Real, bool kMask>
-__device__ inline void eval_pp(const typename Vec4T<Real>::T* e, int C, Real tx, Real ty, Real tz, Real eps2,
-                               Acc<Real>& acc) {
+__device__ inline void eval_pp_part(const typename Vec4T<Real>::T* e, int C, Real tx, Real ty, Real tz, Real eps2,
+                                    Real& bx, Real& by, Real& bz) {
   using V4 = typename Vec4T<Real>::T;
-  Real bx = 0, by = 0, bz = 0;
 #pragma unroll 4
   for (int k = 0; k < C; ++k) {
     const V4 q = e[k];
@@ -216,6 +216,14 @@ __device__ inline void eval_pp(const typename Vec4T<Real>::T* e, int C, Real tx,
     by = fmaR(s3, dy, by);
     bz = fmaR(s3, dz, bz);
   }
+}
+
+// The same, added to the compensated accumulator.
+template <typename Real, bool kMask>
+__device__ inline void eval_pp(const typename Vec4T<Real>::T* e, int C, Real tx, Real ty, Real tz, Real eps2,
+                               Acc<Real>& acc) {
+  Real bx = 0, by = 0, bz = 0;
+  eval_pp_part<Real, kMask>(e, C, tx, ty, tz, eps2, bx, by, bz);
   acc.add(bx, by, bz);
 }
 
@@ -241,13 +249,21 @@ __device__ inline void p2c_term(const V4& e0, const V4& e1, const V2& e2, Real t
   bz = fmaR(inv5, fmaR(g, dz, -qz), bz);
 }
 
-// P2C over this lane's C consecutive cells from j0.
+// P2C over this lane's C consecutive cells from j0, added to the plain
+// partial sums bx, by, bz.
+template <typename Real>
+__device__ inline void eval_pc_part(const PcRing<Real>& r, int j0, int C, Real tx, Real ty, Real tz, Real eps2, Real& bx,
+                                    Real& by, Real& bz) {
+#pragma unroll 2
+  for (int k = 0; k < C; ++k) p2c_term(r.a[j0 + k], r.b[j0 + k], r.c[j0 + k], tx, ty, tz, eps2, bx, by, bz);
+}
+
+// The same, added to the compensated accumulator.
 template <typename Real>
 __device__ inline void eval_pc(const PcRing<Real>& r, int j0, int C, Real tx, Real ty, Real tz, Real eps2,
                                Acc<Real>& acc) {
   Real bx = 0, by = 0, bz = 0;
-#pragma unroll 2
-  for (int k = 0; k < C; ++k) p2c_term(r.a[j0 + k], r.b[j0 + k], r.c[j0 + k], tx, ty, tz, eps2, bx, by, bz);
+  eval_pc_part<Real>(r, j0, C, tx, ty, tz, eps2, bx, by, bz);
   acc.add(bx, by, bz);
 }
 
@@ -299,20 +315,32 @@ __device__ inline void eval_pc_group(const PcRing<Real>& r, int j0, int C, const
 }
 
 // Evaluator of a target's particles: lane = (particle p, slice s), batches
-// of B = S * C entries, slice s takes entries [s*C, (s+1)*C).
+// of B = S * C entries, slice s takes entries [s*C, (s+1)*C). Batches are
+// summed in plain FP32 and folded into the compensated accumulator every
+// kFold batches (small targets have short per-lane batches).
 template <typename Real, bool kMask>
 struct TargetEval {
   using V4 = typename Vec4T<Real>::T;
+  static constexpr int kFold = 2;
   Real tx, ty, tz, eps2;
   int C, s;
   bool active;
   uint32_t B;
+  int pending = 0;
+  Real px = 0, py = 0, pz = 0;
   Acc<Real> acc;
+  __device__ inline void fold() {
+    acc.add(px, py, pz);
+    px = py = pz = 0;
+    pending = 0;
+  }
   __device__ inline void pp(const V4* ring) {
-    if (active) eval_pp<Real, kMask>(&ring[s * C], C, tx, ty, tz, eps2, acc);
+    if (active) eval_pp_part<Real, kMask>(&ring[s * C], C, tx, ty, tz, eps2, px, py, pz);
+    if (++pending == kFold) fold();
   }
   __device__ inline void pc(const PcRing<Real>& ring) {
-    if (active) eval_pc<Real>(ring, s * C, C, tx, ty, tz, eps2, acc);
+    if (active) eval_pc_part<Real>(ring, s * C, C, tx, ty, tz, eps2, px, py, pz);
+    if (++pending == kFold) fold();
   }
 };
 
@@ -769,6 +797,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
                                  n_pc, nullptr, n_def);
     n_pp += g_pp;
     n_pc += g_pc;
+    ev.fold();
     // combine slices in FP64 (the P2C ring is free now)
     double* red = reinterpret_cast<double*>(L.pc.a);  // 3 x 64 doubles
     if (ev.active) {
