@@ -323,7 +323,7 @@ struct TargetEval {
   using V4 = typename Vec4T<Real>::T;
   static constexpr int kFold = 2;
   Real tx, ty, tz, eps2;
-  int C, s;
+  int S, C, s;  // slices, entries per slice and batch, this lane's slice
   bool active;
   uint32_t B;
   int pending = 0;
@@ -334,6 +334,10 @@ struct TargetEval {
     px = py = pz = 0;
     pending = 0;
   }
+  // The last, partial batch of n entries (padded to B): each slice takes
+  // ceil(n / S) entries; restore() returns to full batches.
+  __device__ inline void last_batch(uint32_t n) { C = (static_cast<int>(n) + S - 1) / S; }
+  __device__ inline void restore() { C = static_cast<int>(B) / S; }
   __device__ inline void pp(const V4* ring) {
     if (active) eval_pp_part<Real, kMask>(&ring[s * C], C, tx, ty, tz, eps2, px, py, pz);
     if (++pending == kFold) fold();
@@ -362,6 +366,7 @@ struct GroupEval {
   int nc;          // slots in the fused loop (1 or 2)
   int c0, o0;      // fused loop: entries [o0, o0 + c0) of each batch
   int c1, o1;      // tail slot 1 (c1 > 0): entries [o1, o1 + c1)
+  uint32_t sl0 = 0, sl1 = 0;  // this lane's slices
   bool tail_on;    // this lane's tail slot holds a particle
   uint32_t base[kGroupChunks], rows[kGroupChunks], slices[kGroupChunks];
   uint32_t B;
@@ -403,8 +408,8 @@ struct GroupEval {
       slices[0] = 1;
       slices[1] = S1;
       c1 = kBatch / static_cast<int>(S1);
-      const uint32_t s1 = l / r < S1 ? l / r : S1 - 1;
-      o1 = static_cast<int>(s1) * c1;
+      sl1 = l / r < S1 ? l / r : S1 - 1;
+      o1 = static_cast<int>(sl1) * c1;
       tail_on = l < r * S1;
     } else {
       nc = 2;
@@ -414,9 +419,27 @@ struct GroupEval {
       slices[0] = slices[1] = 1;
     }
     c0 = kBatch / static_cast<int>(slices[0]);
-    const uint32_t s0 = l / rows[0] < slices[0] ? l / rows[0] : slices[0] - 1;
-    o0 = static_cast<int>(s0) * c0;
+    sl0 = l / rows[0] < slices[0] ? l / rows[0] : slices[0] - 1;
+    o0 = static_cast<int>(sl0) * c0;
     B = kBatch;
+  }
+  // The last, partial batch of n entries (padded to B): each slice takes
+  // ceil(n / S) entries; restore() returns to full batches.
+  __device__ inline void last_batch(uint32_t n) {
+    c0 = static_cast<int>((n + slices[0] - 1) / slices[0]);
+    o0 = static_cast<int>(sl0) * c0;
+    if (c1 > 0) {
+      c1 = static_cast<int>((n + slices[1] - 1) / slices[1]);
+      o1 = static_cast<int>(sl1) * c1;
+    }
+  }
+  __device__ inline void restore() {
+    c0 = kBatch / static_cast<int>(slices[0]);
+    o0 = static_cast<int>(sl0) * c0;
+    if (c1 > 0) {
+      c1 = kBatch / static_cast<int>(slices[1]);
+      o1 = static_cast<int>(sl1) * c1;
+    }
   }
   // the particle (index within the group, clamped to it) of slot k
   __device__ inline uint32_t particle(int k, int lane, uint32_t G) const {
@@ -696,11 +719,14 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
     if (om) ++depth;
   }
   if (!ok) return false;
-  // final partial batches (fewer than B entries), padded to B
+  // final partial batches (fewer than B entries), padded to B; each slice
+  // evaluates only its share of the entries actually staged
   if (pp_cnt) {
     if (static_cast<uint32_t>(lane) >= pp_cnt && static_cast<uint32_t>(lane) < B) L.pp[lane] = pad_entry<Real>();
+    ev.last_batch(pp_cnt);
     pp_cnt = B;
     flush_pp();
+    ev.restore();
   }
   if (pc_cnt) {
     if (static_cast<uint32_t>(lane) >= pc_cnt && static_cast<uint32_t>(lane) < B) {
@@ -709,8 +735,10 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
       L.pc.c[lane].x = Real(0);
       L.pc.c[lane].y = Real(0);
     }
+    ev.last_batch(pc_cnt);
     pc_cnt = B;
     flush_pc();
+    ev.restore();
   }
   __builtin_amdgcn_wave_barrier();
   return true;
@@ -780,6 +808,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
     const uint32_t tc = T - c0 < kWave ? T - c0 : kWave;
     const int S = kWave / static_cast<int>(tc) < kBatch ? kWave / static_cast<int>(tc) : kBatch;
     TargetEval<Real, kMask> ev;
+    ev.S = S;
     ev.C = kBatch / S;
     ev.B = static_cast<uint32_t>(S * ev.C);
     ev.active = lane < static_cast<int>(tc) * S;
