@@ -744,6 +744,14 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
   return true;
 }
 
+// The unit's interaction totals, one atomic pair per wavefront.
+__device__ inline void add_totals(const WalkArgs& a, unsigned long long pp, unsigned long long pc) {
+  if (__lane_id() == 0) {
+    atomicAdd(&a.totals[0], pp);
+    atomicAdd(&a.totals[1], pc);
+  }
+}
+
 __device__ inline double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
   return v;
@@ -791,7 +799,8 @@ __device__ inline Box6 node_box(const WalkArgs& a, uint32_t node) {
 // given, times G, to the outputs and the fused kick/drift.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t tnode, const uint32_t* start,
-                            uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc) {
+                            uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc,
+                            unsigned long long& tot_pp, unsigned long long& tot_pc) {
   const int lane = __lane_id();
   const uint32_t tb = uniform(a.begin[tnode]);
   const uint32_t T = uniform(a.count[tnode]);
@@ -873,10 +882,8 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
         }
       }
     }
-    if (lane == 0) {
-      atomicAdd(&a.totals[0], static_cast<unsigned long long>(n_pp) * tc);
-      atomicAdd(&a.totals[1], static_cast<unsigned long long>(n_pc) * tc);
-    }
+    tot_pp += static_cast<unsigned long long>(n_pp) * tc;
+    tot_pc += static_cast<unsigned long long>(n_pc) * tc;
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -901,7 +908,9 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   const bool gleaf = (a.rec[g].count >> 31) != 0;
   uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(unit) * (kLevels - Lds::kFrames) * kWave : nullptr;
   if (!packed && (gleaf || G <= sp.tsm || G > kGroupCap)) {
-    walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0);
+    unsigned long long tot_pp = 0, tot_pc = 0;
+    walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0, tot_pp, tot_pc);
+    add_totals(a, tot_pp, tot_pc);
     return;
   }
   // a packed unit's box and the origin of its relative coordinates (the
@@ -957,6 +966,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
     __threadfence_block();
   }
   // the group's targets, in particle order (marked at their first particle)
+  unsigned long long tot_pp = 0, tot_pc = 0;
   for (uint32_t c = 0; c < G; c += kWave) {
     const uint32_t i = c + lane;
     const uint32_t m = i < G ? a.tmark[gb + i] : 0u;
@@ -966,11 +976,12 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
       bits &= bits - 1;
       const uint32_t tnode = uniform(__builtin_amdgcn_readlane(m, j)) - 1u;
       if (ok)
-        walk_target<Real, kMask>(a, L, gfr, tnode, dl, n_def, a.gacc, g_pp, g_pc);
+        walk_target<Real, kMask>(a, L, gfr, tnode, dl, n_def, a.gacc, g_pp, g_pc, tot_pp, tot_pc);
       else
-        walk_target<Real, kMask>(a, L, gfr, tnode, nullptr, 0, nullptr, 0, 0);
+        walk_target<Real, kMask>(a, L, gfr, tnode, nullptr, 0, nullptr, 0, 0, tot_pp, tot_pc);
     }
   }
+  add_totals(a, tot_pp, tot_pc);
 }
 
 // Block b runs on XCD b % 8; give each XCD a contiguous run of groups so
