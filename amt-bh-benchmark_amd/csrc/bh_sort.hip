@@ -319,22 +319,21 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
 // (x, y, z, m), as sort_particles' final loop does (flat_build_ttg.h:180-188).
 __global__ __launch_bounds__(kBlock) void k_gather(const bh_particle* __restrict__ in, bh_particle* __restrict__ out,
                                                    const uint32_t* __restrict__ perm, double4* __restrict__ posm, uint32_t n) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= n) return;
+  // four lanes per particle, one 16-byte quarter each: consecutive lanes
+  // read and write consecutive 16-byte pieces
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t k = t >> 2, part = t & 3u;
+  if (k >= n) return;  // whole quads leave together
   const uint32_t j = perm[k];
-  const uint4* src = reinterpret_cast<const uint4*>(in + j);
-  uint4* dst = reinterpret_cast<uint4*>(out + k);
-  const uint4 a = src[0], b = src[1], c = src[2], d = src[3];
-  dst[0] = a;
-  dst[1] = b;
-  dst[2] = c;
-  dst[3] = d;
-  double4 q;
-  q.x = __hiloint2double(b.y, b.x);
-  q.y = __hiloint2double(b.w, b.z);
-  q.z = __hiloint2double(c.y, c.x);
-  q.w = __hiloint2double(a.w, a.z);
-  posm[k] = q;
+  const uint4 v = reinterpret_cast<const uint4*>(in + j)[part];
+  reinterpret_cast<uint4*>(out + k)[part] = v;
+  // quarters: 0 = {id, pad, mass}, 1 = {x, y}, 2 = {z, vx}, 3 = {vy, vz}
+  const int lane = __lane_id();
+  const uint32_t mlo = static_cast<uint32_t>(__shfl(static_cast<int>(v.z), lane ^ 2));
+  const uint32_t mhi = static_cast<uint32_t>(__shfl(static_cast<int>(v.w), lane ^ 2));
+  uint4* q = reinterpret_cast<uint4*>(posm + k);
+  if (part == 1) q[0] = v;
+  if (part == 2) q[1] = make_uint4(v.x, v.y, mlo, mhi);
 }
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap) {
@@ -412,7 +411,7 @@ hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_
 
 hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, out, perm, posm, n);
+  hipLaunchKernelGGL(k_gather, dim3((4ull * n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, out, perm, posm, n);
   return hipGetLastError();
 }
 
