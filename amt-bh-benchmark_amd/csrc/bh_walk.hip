@@ -354,7 +354,9 @@ struct TargetEval {
 // kBatch / S_k consecutive entries of each batch). Layouts, by G:
 //   G <= 42      two slots, P = ceil(G / 2), S = the largest divisor of
 //                kBatch with P S <= 64 (one fused loop)
-//   43 .. 64     one slot, P = 64, S = 1
+//   43 .. 53     slot 0: particles 0..31 in 2 slices; slot 1: the r = G - 32
+//                others, S_1 = the largest divisor of kBatch with r S_1 <= 64
+//   54 .. 64     one slot, P = 64, S = 1
 //   65 .. 96     slot 0: particles 0..63, every entry; slot 1: the r = G - 64
 //                others, S_1 = the largest divisor of kBatch with r S_1 <= 64
 //   97 .. 128    two slots, P = 64, S = 1 (one fused loop)
@@ -389,6 +391,24 @@ struct GroupEval {
       base[1] = P;
       rows[0] = rows[1] = P;
       slices[0] = slices[1] = S;
+    } else if (G <= 53) {
+      // particles 0..31 in two slices, the other r = G - 32 in S1 slices
+      const uint32_t r = G - 32;
+      uint32_t S1 = 1;
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        if (S1 == 1 && kDiv[q] * r <= static_cast<uint32_t>(kWave)) S1 = kDiv[q];
+      nc = 1;
+      base[0] = 0;
+      base[1] = 32;
+      rows[0] = 32;
+      rows[1] = r;
+      slices[0] = 2;
+      slices[1] = S1;
+      c1 = kBatch / static_cast<int>(S1);
+      sl1 = l / r < S1 ? l / r : S1 - 1;
+      o1 = static_cast<int>(sl1) * c1;
+      tail_on = l < r * S1;
     } else if (G <= 64) {
       nc = 1;
       base[0] = base[1] = 0;

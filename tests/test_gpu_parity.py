@@ -199,8 +199,9 @@ def walk_group_sizes(export, tsm, cap=128):
 @pytest.mark.parametrize("kind,n,ml", [("plummer", 60000, 32), ("hernquist", 40000, 16), ("plummer", 30000, 8)])
 def test_group_layouts_by_size(gpu, orc, kind, n, ml):
     """The shared phase lays a unit's particles over the lanes by its size
-    (two per lane in slices up to 42, one per lane up to 64, a sliced tail
-    up to 96, two per lane up to 128), and runs of sibling single-target
+    (two per lane in slices up to 42, 32 in two slices plus a sliced rest
+    up to 53, one per lane up to 64, a sliced tail up to 96, two per lane up
+    to 128), and runs of sibling single-target
     groups are packed into units: every range and packed units occur here,
     and lists and forces match the oracle."""
     p = getattr(ic, kind)(n, 13)
@@ -208,7 +209,7 @@ def test_group_layouts_by_size(gpu, orc, kind, n, ml):
     ot = orc.build(p, ml, 256)
     ex = ot.export()
     sizes = walk_group_sizes(ex, 32)
-    for lo, hi in ((33, 42), (43, 64), (65, 96), (97, 128)):
+    for lo, hi in ((33, 42), (43, 53), (54, 64), (65, 96), (97, 128)):
         assert ((sizes >= lo) & (sizes <= hi)).any(), (lo, hi)
     assert packed_runs(ex, 32) > 0
     oa, opp, opc, otot = ot.grouped_pass(fp, 32)
