@@ -291,24 +291,35 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
     if (leaf) {
       const uint32_t b = t.begin[i];
       cnt = t.count[i];
-      for (uint32_t p = b; p < b + cnt; ++p) {
-        const double4 q = posm[p];
-        const double mass = q.w;
-        blo[0] = fmin(blo[0], q.x);
-        blo[1] = fmin(blo[1], q.y);
-        blo[2] = fmin(blo[2], q.z);
-        bhi[0] = fmax(bhi[0], q.x);
-        bhi[1] = fmax(bhi[1], q.y);
-        bhi[2] = fmax(bhi[2], q.z);
-        const double d[3] = {q.x - ref[0], q.y - ref[1], q.z - ref[2]};
-        rm += mass;
-        for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
-        m2[0] += mass * d[0] * d[0];
-        m2[1] += mass * d[1] * d[1];
-        m2[2] += mass * d[2] * d[2];
-        m2[3] += mass * d[0] * d[1];
-        m2[4] += mass * d[0] * d[2];
-        m2[5] += mass * d[1] * d[2];
+      // the particles' loads go out eight at a time (the sums still run in
+      // particle order)
+      constexpr uint32_t kAhead = 8;
+      for (uint32_t p0 = b; p0 < b + cnt; p0 += kAhead) {
+        const uint32_t m = b + cnt - p0 < kAhead ? b + cnt - p0 : kAhead;
+        double4 qs[kAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) qs[u] = posm[p0 + (u < m ? u : 0)];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+          if (u >= m) continue;
+          const double4 q = qs[u];
+          const double mass = q.w;
+          blo[0] = fmin(blo[0], q.x);
+          blo[1] = fmin(blo[1], q.y);
+          blo[2] = fmin(blo[2], q.z);
+          bhi[0] = fmax(bhi[0], q.x);
+          bhi[1] = fmax(bhi[1], q.y);
+          bhi[2] = fmax(bhi[2], q.z);
+          const double d[3] = {q.x - ref[0], q.y - ref[1], q.z - ref[2]};
+          rm += mass;
+          for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
+          m2[0] += mass * d[0] * d[0];
+          m2[1] += mass * d[1] * d[1];
+          m2[2] += mass * d[2] * d[2];
+          m2[3] += mass * d[0] * d[1];
+          m2[4] += mass * d[0] * d[2];
+          m2[5] += mass * d[1] * d[2];
+        }
       }
     } else {
       const uint32_t fc = t.first_child[i], k = t.nchild[i];
