@@ -75,9 +75,13 @@ __global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restric
                                                        const uint16_t* __restrict__ sid, const SysDev* __restrict__ sys,
                                                        uint32_t mlx, uint8_t* __restrict__ L, uint8_t* __restrict__ H,
                                                        uint32_t* __restrict__ tile_counts) {
-  __shared__ uint64_t s_k[kTreeTile + 2 * kMaxLeafCap + 1];
-  __shared__ int8_t s_p[kTreeTile + kMaxLeafCap];
+  // dynamic LDS sized by the batch's largest max_leaf (mlx): keys of
+  // [base - mlx - 1, base + tile + mlx), then P of [base - mlx, base + tile)
+  extern __shared__ __align__(8) unsigned char s_dyn[];
+  uint64_t* s_k = reinterpret_cast<uint64_t*>(s_dyn);
+  int8_t* s_p = reinterpret_cast<int8_t*>(s_k + kTreeTile + 2 * mlx + 1);
   __shared__ uint32_t s_cnt[kTreeCols];
+  __shared__ uint32_t s_one;  // 1 + system id when the tile and its halo hold one system, else 0
   const uint32_t tile = blockIdx.x, base = tile * kTreeTile;
   const int64_t g0 = static_cast<int64_t>(base) - mlx - 1;  // global index of s_k[0]
   const uint32_t wk = kTreeTile + 2 * mlx + 1;
@@ -86,13 +90,19 @@ __global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restric
     s_k[e] = (g >= 0 && g < n) ? keys[g] : 0ull;
   }
   if (threadIdx.x < kTreeCols) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    const int64_t lo = g0 + 1 > 0 ? g0 + 1 : 0;
+    const int64_t hi = static_cast<int64_t>(base) + kTreeTile + mlx < n ? static_cast<int64_t>(base) + kTreeTile + mlx : n - 1;
+    s_one = sid[lo] == sid[hi] ? 1u + sid[lo] : 0u;
+  }
   __syncthreads();
+  const uint32_t one = s_one;
   // P_u for u in [base - mlx, base + tile)
   for (uint32_t e = threadIdx.x; e < kTreeTile + mlx; e += kBlock) {
     const int64_t u = static_cast<int64_t>(base) - mlx + e;
     int8_t pv = -1;
     if (u >= 0 && u < n) {
-      const SysDev& q = sys[sid[u]];
+      const SysDev& q = sys[one ? one - 1u : sid[u]];
       if (u + q.max_leaf < q.off + q.n) pv = static_cast<int8_t>(common_digits(s_k[u - g0], s_k[u + q.max_leaf - g0]));
     }
     s_p[e] = pv;
@@ -101,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restric
   for (uint32_t e = threadIdx.x; e < kTreeTile; e += kBlock) {
     const uint32_t i = base + e;
     if (i >= n) break;
-    const SysDev& q = sys[sid[i]];
+    const SysDev& q = sys[one ? one - 1u : sid[i]];
     const uint32_t ml = q.max_leaf;
     const int li = (i == q.off) ? 0 : common_digits(s_k[i - 1 - g0], s_k[i - g0]) + 1;
     const uint32_t span = (i - q.off < ml) ? i - q.off : ml;  // u = i - d, d <= span, stays in the system
@@ -496,7 +506,8 @@ inline uint32_t grid_for(uint64_t n, uint32_t cap) {
 hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, const uint16_t* sid, const SysDev* sys,
                              uint32_t max_leaf_max, uint8_t* L, uint8_t* H, uint32_t* tile_counts, hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
-  hipLaunchKernelGGL(k_tree_count, dim3(tiles), dim3(kBlock), 0, s, keys, n, sid, sys, max_leaf_max, L, H, tile_counts);
+  const size_t lds = (kTreeTile + 2 * size_t(max_leaf_max) + 1) * sizeof(uint64_t) + kTreeTile + max_leaf_max;
+  hipLaunchKernelGGL(k_tree_count, dim3(tiles), dim3(kBlock), lds, s, keys, n, sid, sys, max_leaf_max, L, H, tile_counts);
   return hipGetLastError();
 }
 
