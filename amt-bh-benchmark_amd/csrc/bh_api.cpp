@@ -1,7 +1,7 @@
 // Host side of the C ABI (include/bh_gpu.h): the context, its grow-only
 // device buffers, and the per-step orchestration of the kernels.
 //
-// One step (step_particles, integrate_ttg.h:111-193, grouped branch):
+// One step (step_particles, integrate_ttg.h:88-107, grouped branch):
 //   bounds -> keys + digit histograms -> 8 onesweep radix passes -> gather
 //   (Morton-ordered particles + FP64 walk copy) -> tree counts -> [one
 //   16-word read-back: node/leaf totals, non-finite flag] -> node emit ->

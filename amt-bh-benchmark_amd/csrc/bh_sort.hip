@@ -2,7 +2,7 @@
 //
 // Reference semantics:
 //   particle_bounds          flat_build_ttg.h:93-113
-//   domain_from_bounds       domain.h:84-91
+//   domain_from_bounds       domain.h:23-30
 //   morton_key/spread_bits3  flat_tree.h:104-125
 //   sort order               flat_build_ttg.h:62-66 (ascending key, ties by input index)
 //   gather to Morton order   flat_build_ttg.h:180-188
@@ -30,7 +30,7 @@ __device__ inline uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// domain.h:84-91 from the reduced bounds lohi = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z}.
+// domain.h:23-30 from the reduced bounds lohi = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z}.
 __device__ inline void domain_from_lohi(const double* lohi, double c[3], double* hw) {
   c[0] = 0.5 * (lohi[0] + lohi[3]);
   c[1] = 0.5 * (lohi[1] + lohi[4]);

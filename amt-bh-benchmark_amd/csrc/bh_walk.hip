@@ -10,7 +10,7 @@
 //   P2P               grouped_ttg.h:124-163: a += m d (r^2 + eps^2)^(-3/2), r = 0 masked
 //                     when eps2 == 0
 //   P2C               multipole.h:171-200: monopole + traceless quadrupole
-//   kick/drift        integrate.h:15-16 via integrate_ttg.h:119-122, fused
+//   kick/drift        integrate.h:15-16 via integrate_ttg.h:74-77, fused
 //
 // Work is split into walk units, one wavefront each: walk groups (topmost
 // nodes with leaf || count <= max(tsm, 128)), with runs of sibling groups

@@ -7,7 +7,7 @@
  * Plain C types only; all device state is owned by a bh_ctx.
  *
  * Reference interfaces replaced (paths relative to the reference repo):
- *   step_particles          ttg-parsec/src/integrate_ttg.h:111-193 (grouped branch, 1 rank)
+ *   step_particles          ttg-parsec/src/integrate_ttg.h:66-148 (grouped branch, 1 rank: :88-107)
  *   build_flat_tree         ttg-parsec/src/flat_build_ttg.h:276-281
  *   run_grouped_pass        ttg-parsec/src/grouped_ttg.h:297-323
  *   brute_force_accel       ttg-parsec/src/brute_force.h:15-28
@@ -41,7 +41,7 @@ typedef struct bh_force_params {
   int32_t pad;
 } bh_force_params;
 
-/* bh::StepTiming (integrate_ttg.h:69-73), milliseconds; exchange_ms is 0 on one GPU. */
+/* bh::StepTiming (integrate_ttg.h:24-28), milliseconds; exchange_ms is 0 on one GPU. */
 typedef struct bh_step_timing {
   double tree_build_ms;
   double traversal_ms;
@@ -62,7 +62,7 @@ typedef struct bh_walk_node {
   uint8_t pad;
 } bh_walk_node;
 
-/* Status codes. The reference throws (integrate_ttg.h:149-151, main.cc:147);
+/* Status codes. The reference throws (integrate_ttg.h:104-106, main.cc:147);
  * no exception crosses this ABI: every call returns one of these and
  * bh_last_error() holds the message. */
 enum bh_status {
@@ -89,7 +89,7 @@ enum bh_layout { BH_LAYOUT_REFERENCE = 0, BH_LAYOUT_BFS = 1 };
 typedef struct bh_ctx bh_ctx;
 
 /* Context on HIP device `device`: one stream, grow-only device buffers (the
- * GroupedWorkspace of integrate_ttg.h:106-109). Not re-entrant: use one
+ * GroupedWorkspace of integrate_ttg.h:61-64). Not re-entrant: use one
  * context per host thread. */
 int bh_ctx_create(bh_ctx** out, int device);
 void bh_ctx_destroy(bh_ctx* ctx);
@@ -105,11 +105,11 @@ int bh_set_walk_mode(bh_ctx* ctx, int mode);
 /* Limits: n < 2^30 particles, max_leaf <= 1024. */
 uint64_t bh_max_particles(void);
 
-/* step_particles (integrate_ttg.h:111-114), grouped branch on one rank:
+/* step_particles (integrate_ttg.h:66-69), grouped branch on one rank:
  * builds the tree of particles[0..n), walks it and applies
  * v += kick_dt * a; if (drift_dt != 0) x += drift_dt * v
  * to every particle, and returns them IN MORTON ORDER in `particles` (as the
- * reference does, integrate_ttg.h:152). Host memory in and out.
+ * reference does, integrate_ttg.h:107). Host memory in and out.
  * group_size must be > 0 (0 selects the reference's per-node path, which
  * this library does not provide). `timing` may be NULL. */
 int bh_step(bh_ctx* ctx, bh_particle* particles, uint64_t n, const bh_force_params* fp, uint32_t max_leaf,
@@ -235,7 +235,7 @@ int bh_accel(bh_ctx* ctx, const bh_force_params* fp, uint32_t target_size, doubl
 
 /* Per-phase device times of the last step or bh_accel call, from HIP events
  * recorded on the context's stream (the reference's StepTiming split,
- * integrate_ttg.h:69-73, refined), and the work the walk did. Has no
+ * integrate_ttg.h:24-28, refined), and the work the walk did. Has no
  * reference counterpart beyond StepTiming; it exists for measurement. */
 typedef struct bh_step_stats {
   double sort_ms;     /* bounds, Morton keys, radix sort, gather to Morton order */

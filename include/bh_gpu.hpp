@@ -5,12 +5,12 @@
 //   reference                                   this header
 //   bh::Particle        particle.h:7-24         bh_gpu::Particle (same 64-byte layout)
 //   bh::ForceParams     traversal_ttg.h:136-141 bh_gpu::ForceParams (same fields, defaults)
-//   bh::StepTiming      integrate_ttg.h:69-73   bh_gpu::StepTiming
-//   bh::GroupedWorkspace integrate_ttg.h:106-109 bh_gpu::GroupedWorkspace (owns the device context)
-//   bh::step_particles  integrate_ttg.h:111-193 bh_gpu::step_particles (grouped branch, one GPU)
+//   bh::StepTiming      integrate_ttg.h:24-28   bh_gpu::StepTiming
+//   bh::GroupedWorkspace integrate_ttg.h:61-64 bh_gpu::GroupedWorkspace (owns the device context)
+//   bh::step_particles  integrate_ttg.h:66-148 bh_gpu::step_particles (grouped branch, one GPU)
 //
 // Error behaviour follows the reference: an incomplete tree throws
-// std::logic_error (integrate_ttg.h:149-151); invalid input throws
+// std::logic_error (integrate_ttg.h:104-106); invalid input throws
 // std::invalid_argument / std::runtime_error (main.cc:147); device failures
 // throw std::runtime_error with bh_last_error()'s message.
 #pragma once
@@ -83,7 +83,7 @@ inline bh_force_params to_c(const ForceParams& fp) {
   return c;
 }
 
-// step_particles (integrate_ttg.h:111-114), grouped branch (group_size > 0):
+// step_particles (integrate_ttg.h:66-69), grouped branch (group_size > 0):
 // builds the tree, computes every particle's acceleration, applies
 // v += kick_dt * a and, if drift_dt != 0, x += drift_dt * v, and returns the
 // particles in Morton order. Without a workspace a temporary context is used.

@@ -103,7 +103,7 @@ void parallel_for(uint64_t n, int threads, F&& f, uint64_t chunk = 1) {
 }
 
 // ---------------------------------------------------------------- domain/key
-// domain.h:84-91
+// domain.h:23-30
 void domain_from_bounds(const double lo[3], const double hi[3], double c[3], double* hw) {
   c[0] = 0.5 * (lo[0] + hi[0]);
   c[1] = 0.5 * (lo[1] + hi[1]);
@@ -723,7 +723,7 @@ void grouped_pass(const Tree& t, const ForceParams& fp, uint32_t target_size, in
   });
 }
 
-// integrate.h:15-16 + integrate_ttg.h:119-122
+// integrate.h:15-16 + integrate_ttg.h:74-77
 inline void kick_drift(Particle& p, const double a[3], double kick_dt, double drift_dt) {
   for (int c = 0; c < 3; ++c) p.vel[c] += a[c] * kick_dt;
   if (drift_dt != 0.0) {
@@ -920,9 +920,9 @@ uint64_t bho_targets(const bho_tree* h, uint32_t target_size, uint32_t* out, uin
   return targets.size();
 }
 
-// integrate_ttg.h:111-193, grouped branch on one rank: build, walk with
+// integrate_ttg.h:88-107, grouped branch on one rank: build, walk with
 // fused kick/drift, return the particles in Morton order (in place).
-// timing3 (nullable): build_ms, walk_ms, 0 (StepTiming, integrate_ttg.h:69-73).
+// timing3 (nullable): build_ms, walk_ms, 0 (StepTiming, integrate_ttg.h:24-28).
 int bho_step(Particle* p, uint64_t n, const ForceParams* fp, uint32_t max_leaf, uint32_t group_size, uint32_t target_size,
              double kick_dt, double drift_dt, int threads, double* timing3) {
   if (n == 0 || n >= (1ull << 31) || group_size == 0 || !finite_input(p, n)) return 1;

@@ -6,7 +6,7 @@
 // repository and nothing here runs on the product path.
 //
 // What runs reference code (compiled from ttg-parsec/src):
-//   bh::compute_domain / domain_from_bounds   (domain.h:84-113)
+//   bh::compute_domain / domain_from_bounds   (domain.h:23-52)
 //   bh::morton_key / spread_bits3             (flat_tree.h:104-125)
 //   bh::detail::init_node / split_node / count_subtree / build_subtree /
 //     finalize_node / is_group_root           (flat_tree.h:182-289)

@@ -185,7 +185,7 @@ def test_golden_fixtures(orc):
 
 # ----------------------------------------------------------------------- walk
 def test_theta0_equals_brute_force(orc):
-    # test_bh_vs_bruteforce.cc:99-119
+    # test_bh_vs_bruteforce.cc:52-76
     p = ic.random_box(200, 1, 10.0)
     fp = layout.force_params(0.0, 1e-6, 1.0, True)
     for gs in (16, 1 << 20):
@@ -201,7 +201,7 @@ def test_brute_force_matches_reference(orc, ref):
 
 
 def test_quadrupole_beats_monopole(orc):
-    # test_bh_vs_bruteforce.cc:122-157: two clusters at +-50
+    # test_bh_vs_bruteforce.cc:78-113: two clusters at +-50
     a = ic.random_box(100, 2, 1.0, center=(-50, 0, 0))
     b = ic.random_box(100, 3, 1.0, center=(50, 0, 0))
     p = np.concatenate([a, b])
@@ -235,7 +235,7 @@ def test_grouped_walk_equals_target_walks(orc, ref, n, ml, gs, ts, quad, eps2):
 
 
 def test_grouped_matches_per_leaf_walk(orc):
-    # test_bh_vs_bruteforce.cc:160-182: target_size <= max_leaf = per-leaf walks
+    # test_bh_vs_bruteforce.cc:116-138: target_size <= max_leaf = per-leaf walks
     p = ic.random_box(2000, 4, 10.0)
     fp = layout.force_params(0.5, 1e-6, 1.0, True)
     t = orc.build(p, 8, 64)
@@ -245,7 +245,7 @@ def test_grouped_matches_per_leaf_walk(orc):
 
 
 def test_shared_targets_not_less_accurate(orc):
-    # test_bh_vs_bruteforce.cc:185-210
+    # test_bh_vs_bruteforce.cc:140-166
     p = ic.random_box(2000, 6, 10.0)
     fp0 = layout.force_params(0.0, 1e-6, 1.0, False)
     t = orc.build(p, 1, 64)
@@ -257,7 +257,7 @@ def test_shared_targets_not_less_accurate(orc):
 
 
 def test_multi_subtree_build(orc):
-    # test_bh_vs_bruteforce.cc:212-242: backbone + subtrees, groups inside / above them
+    # test_bh_vs_bruteforce.cc:168-198: backbone + subtrees, groups inside / above them
     p = ic.random_box(20000, 5, 10.0)
     fp = layout.force_params(0.5, 1e-6, 1.0, True)
     base = orc.build(p, 16, 64).grouped_pass(fp, 32)[0]
