@@ -30,7 +30,16 @@ def test_header_blank_lines_crlf(tmp_path):
 def test_malformed_line_names_it(tmp_path):
     f = tmp_path / "b.csv"
     f.write_text("0,1,2,3,4,5,6,7\n1,1,2,3,4,5,6\n")
-    with pytest.raises(bl.BHError, match="malformed line.*'1,1,2,3,4,5,6'"):
+    with pytest.raises(bl.BHError, match="malformed line 2 of .*'1,1,2,3,4,5,6'"):
+        bl.read_csv(str(f))
+    f.write_text("0,1,2,3,4,5,6,7\n1,1,2,3,4,5,6,7,8\n")  # a ninth field
+    with pytest.raises(bl.BHError, match="malformed line 2 "):
+        bl.read_csv(str(f))
+    f.write_text("id,m,x,y,z,vx,vy,vz\n\n4294967296,1,2,3,4,5,6,7\n")  # id beyond uint32
+    with pytest.raises(bl.BHError, match="malformed line 3 "):
+        bl.read_csv(str(f))
+    f.write_text("0,1,2,3,4,5,6,7x\n")  # trailing garbage in a number
+    with pytest.raises(bl.BHError, match="malformed line 1 "):
         bl.read_csv(str(f))
     with pytest.raises(bl.BHError, match="cannot open"):
         bl.read_csv(str(tmp_path / "missing.csv"))
@@ -54,6 +63,11 @@ def test_large_file_parallel_chunks(tmp_path):
             fh.write(f"{i},{1.0 / n!r},{x!r},{y!r},{z!r},0.0,0.0,0.0\n")
     p = bl.read_csv(str(f))
     assert len(p) == n and np.array_equal(p["id"], np.arange(n)) and np.array_equal(p["pos"], pos)
+    # a bad record far into the file is named by its line number
+    with open(f, "a") as fh:
+        fh.write("1,2,3\n")
+    with pytest.raises(bl.BHError, match=f"malformed line {n + 2} of .*'1,2,3'"):
+        bl.read_csv(str(f))
 
 
 def test_vtk_format(tmp_path):
