@@ -1,17 +1,25 @@
-"""Autotuner for the walk shapes (leaf capacity, group size, target size).
+"""Autotuner for the walk shapes: leaf capacity, group size, target size and
+walk tile.
 
-The three shapes change the tree and the targets' boxes, so they change
-both the work and the interaction lists (hence the accuracy). For each size
-class and opening angle the tuner times every candidate on the GPU (median
-of a few steps of a Plummer and a Hernquist system of that size,
-eps = 0.01) and keeps the fastest whose rms relative acceleration error
-against the FP64 direct sum (seeded subset, worse of the two systems) is
-within the bound; when none is, the most accurate one.
+Leaf capacity, group size and target size change the tree and the targets'
+boxes, so they change both the work and the interaction lists (hence the
+accuracy); the walk tile (particles per walk unit) changes only how targets
+share walks, i.e. the time. For each size class and opening angle the tuner
+measures, on a Plummer and a Hernquist system of that size (eps = 0.01):
+
+- the rms relative acceleration error of every (leaf, group, target) shape
+  against the FP64 direct sum (seeded subset, worse of the two systems);
+- the step time (median of a few steps, summed over the two systems) of
+  every walk tile for each shape within the error bound.
+
+The bound is a hard constraint: the fastest (shape, tile) within it wins.
+Only when no shape reaches it is the most accurate shape kept, with the
+class marked within_bound = false and its error floor recorded.
 
 The table ((size class, theta) -> shapes and the measurements behind the
 choice) is written as JSON next to this module; `shapes_for(n, theta)`
 reads it (nearest size class in log N, nearest theta) and falls back to
-(16, 256, 32) when no table was produced.
+DEFAULT when no table was produced.
 
     python -m bhgpu.autotune [--out PATH] [--sizes 4096,32768,...]
 """
@@ -27,10 +35,12 @@ from . import ic, layout
 from . import lib as bl
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
-DEFAULT = (16, 256, 32)
+MAX_TILE = 128
+DEFAULT = (16, 256, 32, MAX_TILE)  # (max_leaf, group_size, target_size, walk_tile)
 LEAVES = (8, 16, 32, 64)
 GROUPS = (64, 128, 256)
-TARGETS = (16, 32, 64)
+TARGETS = (16, 32, 64, 128, 256)
+TILES = (32, 64, 128)
 
 _cache = {}
 
@@ -45,16 +55,18 @@ def load_table(path=TABLE):
 
 
 def shapes_for(n, theta=0.5, path=TABLE):
-    """(max_leaf, group_size, target_size) for a system of n particles."""
+    """(max_leaf, group_size, target_size, walk_tile) for a system of n
+    particles; tables without a tile give the largest tile."""
     t = load_table(path)
     if not t or not t.get("classes"):
         return DEFAULT
     best = min(t["classes"], key=lambda c: (abs(np.log(max(n, 1)) - np.log(c["n"])), abs(theta - c.get("theta", theta))))
-    return tuple(best["shapes"])
+    sh = tuple(best["shapes"])
+    return sh if len(sh) == 4 else sh + (MAX_TILE,)
 
 
 def _accuracy(ctx, p, fp, shapes, subset=256):
-    ml, gs, ts = shapes
+    ml, gs, ts = shapes[:3]
     ctx.build_tree(p, ml, gs)
     acc, _, _, _ = ctx.accel(fp, ts)
     perm = ctx.export_perm()
@@ -68,14 +80,18 @@ def _accuracy(ctx, p, fp, shapes, subset=256):
 
 
 def _time(ctx, p, fp, shapes, reps=3):
-    ml, gs, ts = shapes
-    ctx.state_upload(p)
-    ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)  # warm-up, grows buffers
-    ts_ms = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)
-        ts_ms.append(1e3 * (time.perf_counter() - t0))
+    ml, gs, ts, tile = shapes
+    ctx.set_walk_tile(tile)
+    try:
+        ctx.state_upload(p)
+        ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)  # warm-up, grows buffers
+        ts_ms = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)
+            ts_ms.append(1e3 * (time.perf_counter() - t0))
+    finally:
+        ctx.set_walk_tile(MAX_TILE)
     return float(np.median(ts_ms))
 
 
@@ -85,25 +101,28 @@ def tune(ctx, sizes, thetas=(0.4, 0.5, 0.7), eps=0.01, bound=1e-3, log=print):
         systems = [ic.plummer(n, 31), ic.hernquist(n, 32)]
         for theta in thetas:
             fp = layout.force_params(theta, eps * eps, 1.0, True)
+            shapes = [(ml, gs, ts) for ml in LEAVES for gs in GROUPS for ts in TARGETS if ts <= gs]
+            errs = {sh: max(_accuracy(ctx, p, fp, sh) for p in systems) for sh in shapes}
+            ok = [sh for sh in shapes if errs[sh] <= bound]
+            timed = ok if ok else [min(shapes, key=lambda sh: errs[sh])]
             rows = []
-            for ml in LEAVES:
-                for gs in GROUPS:
-                    for ts in TARGETS:
-                        if ts > gs:
-                            continue
-                        sh = (ml, gs, ts)
-                        ms = sum(_time(ctx, p, fp, sh) for p in systems)
-                        err = max(_accuracy(ctx, p, fp, sh) for p in systems)
-                        rows.append({"shapes": sh, "ms": ms, "rms_rel": err})
-            ok = [r for r in rows if r["rms_rel"] <= bound]
-            best = min(ok, key=lambda r: r["ms"]) if ok else min(rows, key=lambda r: r["rms_rel"])
-            dflt = next((r for r in rows if tuple(r["shapes"]) == DEFAULT), None)
+            for sh in timed:
+                for tile in TILES:
+                    ms = sum(_time(ctx, p, fp, sh + (tile,)) for p in systems)
+                    rows.append({"shapes": list(sh) + [tile], "ms": ms, "rms_rel": errs[sh]})
+            best = min(rows, key=lambda r: r["ms"])
+            dflt_err = errs.get(DEFAULT[:3])
             log(f"n={n} theta={theta}: best {best['shapes']} {best['ms']:.3f} ms (rms {best['rms_rel']:.2e}, "
-                f"{len(ok)} of {len(rows)} within {bound:g}); default {dflt['ms']:.3f} ms (rms {dflt['rms_rel']:.2e})")
-            classes.append({"n": int(n), "theta": theta, "shapes": list(best["shapes"]), "ms": best["ms"],
-                            "rms_rel": best["rms_rel"], "within_bound": bool(ok), "default_ms": dflt["ms"],
-                            "default_rms_rel": dflt["rms_rel"], "candidates": rows})
-    return {"eps": eps, "bound": bound, "measure": "median of 3 steps each of a Plummer and a Hernquist system",
+                f"{len(ok)} of {len(shapes)} shapes within {bound:g}; error floor {min(errs.values()):.2e})")
+            classes.append({"n": int(n), "theta": theta, "shapes": best["shapes"], "ms": best["ms"],
+                            "rms_rel": best["rms_rel"], "within_bound": bool(ok),
+                            "rms_floor": min(errs.values()), "default_rms_rel": dflt_err,
+                            "shape_errors": [{"shapes": list(sh), "rms_rel": e} for sh, e in errs.items()],
+                            "timings": rows})
+    return {"eps": eps, "bound": bound,
+            "measure": "rms: worse of a Plummer and a Hernquist system (FP64 direct sum, 256 particles); "
+                       "time: median of 3 steps, summed over the two systems, for every walk tile of every "
+                       "shape within the bound",
             "classes": classes}
 
 
@@ -119,7 +138,7 @@ def main(argv=None):
     ctx.close()
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
-    print(json.dumps({"classes": [{k: c[k] for k in ("n", "theta", "shapes", "ms", "rms_rel", "default_ms")}
+    print(json.dumps({"classes": [{k: c[k] for k in ("n", "theta", "shapes", "ms", "rms_rel", "within_bound", "rms_floor")}
                                   for c in table["classes"]]}))
     return 0
 

@@ -32,9 +32,13 @@ def make(count=64, n_min=4096, n_max=1 << 20, eps=0.01):
 
 
 def table(meta, shapes, kick_dt, drift_dt):
-    """bh_system records; shapes(meta entry) -> (max_leaf, group_size, target_size)."""
+    """bh_system records; shapes(meta entry) -> (max_leaf, group_size,
+    target_size[, walk_tile])."""
     rows = []
     for m in meta:
-        ml, gs, ts = shapes(m)
-        rows.append(system(m["n"], force_params(m["theta"], m["eps2"], 1.0, True), ml, gs, ts, kick_dt, drift_dt))
+        sh = tuple(shapes(m))
+        ml, gs, ts = sh[:3]
+        tile = sh[3] if len(sh) > 3 else 0
+        rows.append(system(m["n"], force_params(m["theta"], m["eps2"], 1.0, True), ml, gs, ts, kick_dt, drift_dt,
+                           walk_tile=tile))
     return np.concatenate(rows) if rows else np.zeros(0, SYSTEM)

@@ -39,15 +39,16 @@ assert FORCE_PARAMS.itemsize == 32
 
 SYSTEM = np.dtype(
     [("n", "<u8"), ("fp", FORCE_PARAMS), ("max_leaf", "<u4"), ("group_size", "<u4"), ("target_size", "<u4"),
-     ("pad", "<u4"), ("kick_dt", "<f8"), ("drift_dt", "<f8")],
+     ("walk_tile", "<u4"), ("kick_dt", "<f8"), ("drift_dt", "<f8")],
     align=True,
 )
 assert SYSTEM.itemsize == 72
 
 
-def system(n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
-    """One bh_system record."""
+def system(n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, walk_tile=0):
+    """One bh_system record (walk_tile 0: the context's)."""
     s = np.zeros(1, SYSTEM)
+    s["walk_tile"] = walk_tile
     s["n"] = n
     s["fp"] = fp[0] if isinstance(fp, np.ndarray) else fp
     s["max_leaf"], s["group_size"], s["target_size"] = max_leaf, group_size, target_size

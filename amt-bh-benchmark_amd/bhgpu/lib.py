@@ -27,6 +27,7 @@ STATUS = {
 BH_FP32, BH_FP64 = 0, 1
 BH_WALK_GROUPED, BH_WALK_PER_TARGET = 0, 1
 BH_LAYOUT_REFERENCE, BH_LAYOUT_BFS = 0, 1
+BH_GRAPH_OFF, BH_GRAPH_ON = 0, 1
 
 # Every entry point of include/bh_gpu.h with its ctypes signature.
 _P = ctypes.c_void_p
@@ -40,6 +41,9 @@ SIGNATURES = {
     "bh_last_error": ([_P], ctypes.c_char_p),
     "bh_set_precision": ([_P, _I], _I),
     "bh_set_walk_mode": ([_P, _I], _I),
+    "bh_set_graph_mode": ([_P, _I], _I),
+    "bh_set_walk_tile": ([_P, _U32], _I),
+    "bh_graph_count": ([_P, ctypes.POINTER(_U32)], _I),
     "bh_max_particles": ([], _U64),
     "bh_step": ([_P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_step_device": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
@@ -102,7 +106,9 @@ class StepTiming(ctypes.Structure):
 class StepStats(ctypes.Structure):
     _fields_ = [("sort_ms", ctypes.c_double), ("tree_ms", ctypes.c_double), ("targets_ms", ctypes.c_double),
                 ("walk_ms", ctypes.c_double), ("n_pp", ctypes.c_uint64), ("n_pc", ctypes.c_uint64),
-                ("n_nodes", ctypes.c_uint64), ("n_leaves", ctypes.c_uint64), ("n_targets", ctypes.c_uint64)]
+                ("n_nodes", ctypes.c_uint64), ("n_leaves", ctypes.c_uint64), ("n_targets", ctypes.c_uint64),
+                ("step_ms", ctypes.c_double), ("n_units", ctypes.c_uint64), ("graph", ctypes.c_uint32),
+                ("reruns", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -212,6 +218,17 @@ class Context:
     def set_walk_mode(self, mode):
         self._check(self._lib.bh_set_walk_mode(self._h, mode))
 
+    def set_graph_mode(self, mode):
+        self._check(self._lib.bh_set_graph_mode(self._h, mode))
+
+    def set_walk_tile(self, tile):
+        self._check(self._lib.bh_set_walk_tile(self._h, tile))
+
+    def graph_count(self):
+        k = _U32()
+        self._check(self._lib.bh_graph_count(self._h, ctypes.byref(k)))
+        return k.value
+
     def step(self, p, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
         """step_particles: returns (particles in Morton order, StepTiming)."""
         out = np.array(p, dtype=PARTICLE, copy=True)
@@ -246,7 +263,7 @@ class Context:
     def state_upload(self, p):
         p = np.ascontiguousarray(p, dtype=PARTICLE)
         self._check(self._lib.bh_state_upload(self._h, _ptr(p), len(p)))
-        self._n = len(p)
+        self._state_n = len(p)
 
     def state_step(self, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
         t = StepTiming()
@@ -261,7 +278,7 @@ class Context:
         return t
 
     def state_download(self):
-        out = np.zeros(self._n, PARTICLE)
+        out = np.zeros(self._state_n, PARTICLE)
         self._check(self._lib.bh_state_download(self._h, _ptr(out), len(out)))
         return out
 
@@ -271,7 +288,7 @@ class Context:
     def build_tree(self, p, max_leaf, group_size):
         p = np.ascontiguousarray(p, dtype=PARTICLE)
         self._check(self._lib.bh_build_tree(self._h, _ptr(p), len(p), max_leaf, group_size))
-        self._n = len(p)
+        self._tree_n = len(p)
 
     def tree_sizes(self):
         nn, ng = _U64(), _U64()
@@ -282,7 +299,7 @@ class Context:
     def export(self, layout=BH_LAYOUT_REFERENCE):
         """The last built tree, as the oracle's export (reference node layout by default)."""
         m, g, dom = self.tree_sizes()
-        n = self._n
+        n = self._tree_n
         out = dict(
             keys=np.zeros(n, np.uint64),
             perm=np.zeros(n, np.uint32),
@@ -300,13 +317,13 @@ class Context:
 
     def export_perm(self):
         """Sorted position -> input index of the last built tree."""
-        perm = np.zeros(self._n, np.uint32)
+        perm = np.zeros(self._tree_n, np.uint32)
         self._check(self._lib.bh_tree_export(self._h, BH_LAYOUT_BFS, None, _ptr(perm), None, None, None, None, None))
         return perm
 
     def accel(self, fp, target_size):
         """run_grouped_pass on the last tree: (acc, n_pp, n_pc, totals), Morton order."""
-        n = self._n
+        n = self._tree_n
         acc = np.zeros((n, 3))
         npp = np.zeros(n, np.uint32)
         npc = np.zeros(n, np.uint32)

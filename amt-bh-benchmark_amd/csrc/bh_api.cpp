@@ -2,11 +2,20 @@
 // device buffers, and the per-step orchestration of the kernels.
 //
 // One step (step_particles, integrate_ttg.h:88-107, grouped branch):
-//   bounds -> keys + digit histograms -> 8 onesweep radix passes -> gather
-//   (Morton-ordered particles + FP64 walk copy) -> tree counts -> [one
-//   16-word read-back: node/leaf totals, non-finite flag] -> node emit ->
-//   links/centres -> upward pass (per level) -> targets -> walk with fused
-//   kick/drift. Everything is ordered on the context's stream.
+//   system table upload -> bounds -> keys + digit histograms -> 8 onesweep
+//   radix passes -> gather (Morton-ordered particles + FP64 walk copy) ->
+//   tree counts -> node emit -> links/centres -> upward pass (per level) ->
+//   targets and walk units -> walk with fused kick/drift.
+// Everything is ordered on the context's stream and nothing in between
+// needs the host: the kernels read the node, level and unit counts on the
+// device. The host waits once, at the end, and reads back one block of flag
+// and count words. The node arrays and the number of upward levels are
+// planned from what the context has seen (and from n); a tree that turns out
+// larger or deeper sets an abort word, every kernel after the node count
+// returns at once, and the step runs again with the arrays grown (its input
+// is never written, so running it again is exact). Because a step makes no
+// host decision midway, the whole launch sequence can be captured once into
+// a HIP graph and replayed (bh_set_graph_mode).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,12 +24,15 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "bh_internal.h"
 
 using namespace bhk;
+
+static_assert(kGroupCap == BH_MAX_WALK_TILE && kMinTile == BH_MIN_WALK_TILE, "walk tile limits");
 
 namespace {
 
@@ -30,61 +42,120 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// Every array a step uses besides the particles in and out.
+struct Scratch {
+  uint64_t *keys_a = nullptr, *keys_b = nullptr;
+  uint32_t *vals_a = nullptr, *vals_b = nullptr;
+  uint32_t *hist = nullptr, *dbase = nullptr, *lookback = nullptr;
+  double *lohi = nullptr, *partial = nullptr;
+  uint32_t* misc = nullptr;              // kMiscWords
+  unsigned long long* totals = nullptr;  // [2] interaction totals
+  double4* posm = nullptr;
+  uint8_t *L = nullptr, *H = nullptr;
+  uint32_t *tcounts = nullptr, *tprefix = nullptr;
+  uint16_t* sid = nullptr;
+  uint8_t* tables = nullptr;
+  TreeDev t{};
+  uint32_t *mark = nullptr, *targets = nullptr, *gmark = nullptr, *groups = nullptr, *unit_n = nullptr;
+  uint32_t *defer = nullptr, *deep_frames = nullptr;
+  double* gacc = nullptr;
+};
+
+// What the host decides before a step is enqueued.
+struct Plan {
+  uint32_t n = 0;         // particles of all systems
+  uint32_t S = 0;         // non-empty systems
+  uint32_t n_chunks = 0;  // bounds blocks
+  uint32_t mlx = 0;       // largest max_leaf
+  int sid_passes = 0;     // radix passes over the system id
+  bool unsoftened = false;
+  size_t table_bytes = 0, off_chunks = 0, off_first = 0, off_hist = 0;
+  uint64_t node_estimate = 0;
+  uint32_t cap_nodes = 0;  // node array capacity the kernels may use
+  uint32_t depth_plan = kDepthCap;
+  bool deep = true;        // walk stack frames beyond LDS
+  uint32_t slots = 0;      // walk workgroups
+  bool walk = true;        // false: tree build only
+};
+
+// One captured step, keyed by everything its launches depend on.
+struct GraphKey {
+  const void* in;
+  const void* out;
+  uint64_t gen;
+  uint32_t n, S, n_chunks, mlx, cap_nodes, depth_plan, slots;
+  uint32_t table_bytes;
+  uint8_t deep, unsoftened, precision, mode;
+  bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof *this) == 0; }
+};
+struct GraphEntry {
+  GraphKey key;
+  hipGraphExec_t exec;
+  uint64_t used;
+};
+constexpr size_t kGraphCache = 256;
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  // events: 0 step start, 1 sorted, 2 tree built, 3 targets chosen, 4 walk start, 5 walk end
+  // events: 0 step start, 1 sorted, 2 tree built, 3 targets chosen, 4 walk start, 5 walk (step) end
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   bool have_stats = false;
+  bool last_graph = false;  // the last step ran as a graph: only its total time is known
+  uint32_t reruns = 0;      // times the last step ran again with larger arrays
   std::string err;
   int precision = BH_FP32;
   int walk_mode = BH_WALK_GROUPED;
+  int graph_mode = BH_GRAPH_OFF;
+  uint32_t walk_tile = kGroupCap;
 
-  DevBuf<bh_particle> pa, pb;   // particle ping-pong (resident state / step output)
-  int cur = 0;                  // which of pa/pb holds the resident state
+  DevBuf<bh_particle> pa, pb;    // particle ping-pong of the resident state
+  int cur = 0;                   // which of pa/pb holds the resident state
   uint64_t state_n = 0;
-  DevBuf<bh_particle> tmp_in;   // host-upload staging for bh_step / bh_build_tree
+  DevBuf<bh_particle> tmp_in, tmp_out;  // host-path staging (bh_step, bh_step_batch_host, bh_build_tree)
   DevBuf<uint64_t> keys_a, keys_b;
   DevBuf<uint32_t> vals_a, vals_b;
   DevBuf<uint32_t> hist, dbase, lookback;
   DevBuf<double> lohi, partial;
-  DevBuf<uint32_t> misc;        // [0] non-finite flag, [8..32) tree totals, [40] n_targets
+  DevBuf<uint32_t> misc;
   DevBuf<unsigned long long> totals;
   DevBuf<double4> posm;
   DevBuf<uint8_t> L, H;
   DevBuf<uint32_t> tcounts, tprefix;
-  DevBuf<uint32_t> mark, targets;   // walk targets: marks per particle, list
-  DevBuf<uint32_t> gmark, groups;   // walk groups: marks per particle, list
-  DevBuf<uint32_t> defer;           // deferred nodes per group
-  DevBuf<uint32_t> deep_frames;     // walk stack frames of deep trees, per group
-  DevBuf<double> gacc;              // group sums per particle
-  DevBuf<double> nbox;              // particle box per node
-  DevBuf<uint32_t> unit_n;          // walk units packed from single-target groups, per node
-  // tree (level-order BFS)
+  DevBuf<uint32_t> mark, targets, gmark, groups, unit_n, defer, deep_frames;
+  DevBuf<double> gacc, nbox;
   DevBuf<uint32_t> begin, count, cs, first_child, parent, leaf_idx, leaf_node, leaf_first, level_off;
   DevBuf<uint8_t> depth, leaf, nchild;
   DevBuf<double4> ctr;
   DevBuf<NodeRec> rec;
   DevBuf<double> mass, com, iab, q64;
-  // direct sum
+  DevBuf<uint8_t> tables;
+  DevBuf<uint16_t> sid;
+  // direct sum and energies
   DevBuf<double4> dsrc;
-  DevBuf<uint32_t> didx;
+  DevBuf<uint32_t> didx, dcnt;
   DevBuf<double> dpart, dacc;
 
-  uint32_t*  host_misc = nullptr;   // pinned
-  // system table of the current batch: device copy and pinned staging
-  DevBuf<uint8_t> tables;           // SysDev[S] | uint4 chunks[C] | uint32 chunk_first[S + 1] | uint32 sid hist[2][256]
-  uint8_t* host_tables = nullptr;
+  uint32_t* host_misc = nullptr;  // pinned: kMiscWords flag/count words, then the two totals
+  uint8_t* host_tables = nullptr; // pinned system table staging
   size_t host_tables_cap = 0;
-  DevBuf<uint16_t> sid;
-  uint32_t n_sys = 0, n_chunks = 0, mlx = 0;
-  int sid_passes = 0;
-  bool any_unsoftened = false;
-  size_t off_chunks = 0, off_chunk_first = 0, off_sid_hist = 0;
-  // last built tree
+  uint64_t gen = 1;               // bumped whenever a buffer a graph may hold moves
+
+  // plans learnt from earlier steps
+  uint32_t cap_nodes = 0;        // node capacity asked for (grow-only)
+  bool depth_known = false;
+  uint32_t depth_hint = kDepthCap;
+
+  std::vector<GraphEntry> graphs;
+  uint64_t graph_clock = 0;
+
+  // last tree (for export and bh_accel) and last step's report
   bool has_tree = false;
-  uint32_t n = 0, n_nodes = 0, n_leaves = 0, max_depth = 0;
+  Plan plan{};
+  Scratch sc{};
+  uint32_t n = 0, n_nodes = 0, n_leaves = 0, max_depth = 0, n_targets = 0, n_groups = 0;
   uint32_t max_leaf = 0, group_size = 0;
+  unsigned long long last_totals[2] = {0, 0};
   bh_particle* sorted = nullptr;    // device: Morton-ordered particles of the last tree
   uint64_t* keys_sorted = nullptr;  // device: sorted keys of the last tree
   uint32_t* perm_sorted = nullptr;  // device: sorted position -> input index
@@ -96,6 +167,7 @@ struct HostSys {
   bh_force_params fp;
   uint32_t max_leaf, group_size, target_size;
   double kick_dt, drift_dt;
+  uint32_t tile;  // 0: the context's
 };
 
 bool set_err(Ctx* c, const char* fmt, ...) {
@@ -117,11 +189,28 @@ bool set_err(Ctx* c, const char* fmt, ...) {
     }                                                                                           \
   } while (0)
 
+#define TRY(expr)                 \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != BH_OK) return rc_; \
+  } while (0)
+
+void drop_graphs(Ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
 template <typename T>
 int ensure(Ctx* c, DevBuf<T>& b, size_t n, const char* what) {
   if (n == 0) n = 1;
   if (b.cap >= n) return BH_OK;
-  if (b.p) (void)hipFree(b.p);
+  // a graph may still refer to the old buffer: forget every graph first
+  drop_graphs(c);
+  ++c->gen;
+  if (b.p) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(b.p);
+  }
   b.p = nullptr;
   b.cap = 0;
   const size_t want = n + n / 8;
@@ -142,13 +231,6 @@ void release(DevBuf<T>& b) {
   b.p = nullptr;
   b.cap = 0;
 }
-
-#define TRY(expr)            \
-  do {                       \
-    int rc_ = (expr);        \
-    if (rc_ != BH_OK) return rc_; \
-  } while (0)
-
 
 int check_args(Ctx* c, uint64_t n, const bh_force_params* fp, uint32_t max_leaf, uint32_t group_size) {
   if (n > kMaxN) {
@@ -171,8 +253,209 @@ int check_args(Ctx* c, uint64_t n, const bh_force_params* fp, uint32_t max_leaf,
   return BH_OK;
 }
 
-TreeDev tree_view(Ctx* c) {
-  TreeDev t;
+// Pinned staging of the system table; with no_grow it must already be
+// large enough.
+int host_table_room(Ctx* c, size_t bytes, bool no_grow) {
+  if (c->host_tables_cap >= bytes) return BH_OK;
+  if (no_grow) {
+    set_err(c, "the system table needs %zu bytes of pinned staging, %zu were set up", bytes, c->host_tables_cap);
+    return BH_ERR_TOO_LARGE;
+  }
+  drop_graphs(c);  // graphs copy the table from this buffer
+  ++c->gen;
+  if (c->host_tables) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipHostFree(c->host_tables);
+  }
+  c->host_tables = nullptr;
+  c->host_tables_cap = 0;
+  const size_t want = bytes + bytes / 2;
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->host_tables), want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    set_err(c, "pinned host allocation of %zu bytes failed", want);
+    return BH_ERR_OOM;
+  }
+  c->host_tables_cap = want;
+  return BH_OK;
+}
+
+// The batch's system table: per-system parameters, chunks of <= kChunk
+// particles for the per-system bounds, and the system-id digit counts of the
+// sort's last passes, written to the pinned staging (the step uploads it
+// with one copy). Empty systems take no part: the device table lists the
+// others, in order, and a particle's system id indexes that list.
+int plan_systems(Ctx* c, const HostSys* sys, uint32_t S, bool no_grow, Plan* p) {
+  if (S == 0 || S > kMaxSystems) {
+    set_err(c, "a batch holds 1..%u systems, %u given", kMaxSystems, S);
+    return BH_ERR_INVALID;
+  }
+  uint64_t n = 0, chunks = 0;
+  uint32_t mlx = 0, live = 0;
+  bool unsoftened = false;
+  double nodes = 0.0;
+  for (uint32_t k = 0; k < S; ++k) {
+    TRY(check_args(c, sys[k].n, &sys[k].fp, sys[k].max_leaf, sys[k].group_size));
+    if (sys[k].tile && (sys[k].tile < kMinTile || sys[k].tile > kGroupCap)) {
+      set_err(c, "walk tile %u outside %u..%u", sys[k].tile, kMinTile, kGroupCap);
+      return BH_ERR_INVALID;
+    }
+    if (sys[k].n == 0) continue;
+    ++live;
+    n += sys[k].n;
+    chunks += (sys[k].n + kChunk - 1) / kChunk;
+    mlx = std::max(mlx, sys[k].max_leaf);
+    unsoftened |= static_cast<float>(sys[k].fp.eps2) == 0.0f;  // zero in the FP32 walk too
+    // nodes per particle of Plummer, Hernquist and uniform trees: at most
+    // min(1.6, 4 / max_leaf), with a quarter to spare
+    nodes += 1.25 * static_cast<double>(sys[k].n) * std::min(1.6, 4.0 / std::max(1u, sys[k].max_leaf)) + 64.0;
+  }
+  if (n > kMaxN) {
+    set_err(c, "batch of %llu particles exceeds the supported %llu", (unsigned long long)n, (unsigned long long)kMaxN);
+    return BH_ERR_TOO_LARGE;
+  }
+  *p = Plan{};
+  p->n = static_cast<uint32_t>(n);
+  if (n == 0) return BH_OK;
+  p->S = live;
+  p->n_chunks = static_cast<uint32_t>(chunks);
+  p->mlx = mlx;
+  p->sid_passes = live == 1 ? 0 : (live <= 256 ? 1 : 2);
+  p->unsoftened = unsoftened;
+  p->node_estimate = static_cast<uint64_t>(nodes);
+  p->off_chunks = sizeof(SysDev) * live;
+  p->off_first = p->off_chunks + sizeof(uint4) * chunks;
+  p->off_hist = (p->off_first + 4 * (live + 1) + 15) / 16 * 16;
+  p->table_bytes = p->off_hist + 2 * 256 * 4;
+  TRY(host_table_room(c, p->table_bytes, no_grow));
+  std::memset(c->host_tables, 0, p->table_bytes);
+  SysDev* sd = reinterpret_cast<SysDev*>(c->host_tables);
+  uint4* ch = reinterpret_cast<uint4*>(c->host_tables + p->off_chunks);
+  uint32_t* first = reinterpret_cast<uint32_t*>(c->host_tables + p->off_first);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(c->host_tables + p->off_hist);
+  uint64_t off = 0, ci = 0;
+  for (uint32_t k = 0, src = 0; k < live; ++k, ++src) {
+    while (sys[src].n == 0) ++src;
+    const HostSys& h = sys[src];
+    sd[k].theta2 = h.fp.theta * h.fp.theta;
+    sd[k].eps2 = h.fp.eps2;
+    sd[k].G = h.fp.G;
+    sd[k].kick_dt = h.kick_dt;
+    sd[k].drift_dt = h.drift_dt;
+    sd[k].off = static_cast<uint32_t>(off);
+    sd[k].n = static_cast<uint32_t>(h.n);
+    sd[k].max_leaf = h.max_leaf;
+    sd[k].tsm = std::min(h.target_size, h.group_size);
+    sd[k].use_quad = h.fp.use_quadrupole != 0;
+    sd[k].tile = h.tile ? h.tile : c->walk_tile;
+    first[k] = static_cast<uint32_t>(ci);
+    for (uint64_t b = 0; b < h.n; b += kChunk)
+      ch[ci++] = make_uint4(k, static_cast<uint32_t>(off + b), static_cast<uint32_t>(off + std::min<uint64_t>(h.n, b + kChunk)), 0);
+    hist[k & 255] += static_cast<uint32_t>(h.n);
+    hist[256 + (k >> 8)] += static_cast<uint32_t>(h.n);
+    off += h.n;
+  }
+  first[live] = static_cast<uint32_t>(ci);
+  // single-system queries (export, accel) read the first system's shape
+  for (uint32_t k = 0; k < S; ++k) {
+    if (sys[k].n == 0) continue;
+    c->max_leaf = sys[k].max_leaf;
+    c->group_size = sys[k].group_size;
+    break;
+  }
+  return BH_OK;
+}
+
+// Walk workgroups: enough to fill the GPU (4 waves on every SIMD), fewer for
+// small inputs; always a multiple of 8 (one share per XCD).
+uint32_t walk_slots(uint32_t n, uint32_t limit) {
+  uint32_t s = (n + 7) / 8;
+  s = std::min(s, limit);
+  s = std::max(s, 8u);
+  return (s + 7) / 8 * 8;
+}
+
+// The plan's sizes that depend on what the context has learnt.
+void plan_shape(Ctx* c, Plan* p, bool walk) {
+  p->walk = walk;
+  const uint64_t want = std::max<uint64_t>(p->node_estimate, c->cap_nodes);
+  p->cap_nodes = static_cast<uint32_t>(std::min<uint64_t>(want, 1ull << 30));
+  p->depth_plan = c->depth_known ? c->depth_hint : static_cast<uint32_t>(kDepthCap);
+  p->deep = p->depth_plan + 1 > kShallowFrames;
+  p->slots = walk_slots(p->n, kWalkSlots);
+}
+
+// Grows the context's buffers to the plan and points `b` at them.
+int provision(Ctx* c, const Plan& p, Scratch* b) {
+  const uint32_t n = p.n, m = p.cap_nodes;
+  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
+  TRY(ensure(c, c->keys_a, n, "keys"));
+  TRY(ensure(c, c->keys_b, n, "keys"));
+  TRY(ensure(c, c->vals_a, n, "indices"));
+  TRY(ensure(c, c->vals_b, n, "indices"));
+  TRY(ensure(c, c->hist, 10 * 256, "histograms"));
+  TRY(ensure(c, c->dbase, 10 * 256, "digit offsets"));
+  TRY(ensure(c, c->lookback, radix_lookback_words(n), "look-back"));
+  TRY(ensure(c, c->lohi, 6ull * p.S, "bounds"));
+  TRY(ensure(c, c->partial, 6ull * p.n_chunks, "bounds"));
+  TRY(ensure(c, c->misc, kMiscWords, "flags"));
+  TRY(ensure(c, c->totals, 2, "totals"));
+  TRY(ensure(c, c->posm, n, "walk copy"));
+  TRY(ensure(c, c->L, n, "tree build"));
+  TRY(ensure(c, c->H, n, "tree build"));
+  TRY(ensure(c, c->tcounts, uint64_t(tiles) * kTreeCols, "tree build"));
+  TRY(ensure(c, c->tprefix, uint64_t(tiles) * kTreeCols, "tree build"));
+  TRY(ensure(c, c->level_off, kLevels + 2, "tree levels"));
+  TRY(ensure(c, c->sid, n, "system ids"));
+  TRY(ensure(c, c->tables, p.table_bytes, "system table"));
+  TRY(ensure(c, c->begin, m, "nodes"));
+  TRY(ensure(c, c->count, m, "nodes"));
+  TRY(ensure(c, c->cs, m, "nodes"));
+  TRY(ensure(c, c->first_child, m, "nodes"));
+  TRY(ensure(c, c->parent, m, "nodes"));
+  TRY(ensure(c, c->leaf_idx, m, "nodes"));
+  TRY(ensure(c, c->depth, m, "nodes"));
+  TRY(ensure(c, c->leaf, m, "nodes"));
+  TRY(ensure(c, c->nchild, m, "nodes"));
+  TRY(ensure(c, c->ctr, m, "nodes"));
+  TRY(ensure(c, c->rec, m, "nodes"));
+  TRY(ensure(c, c->mass, m, "nodes"));
+  TRY(ensure(c, c->com, 3ull * m, "nodes"));
+  TRY(ensure(c, c->iab, 6ull * m, "nodes"));
+  TRY(ensure(c, c->q64, 8ull * m, "nodes"));
+  TRY(ensure(c, c->nbox, 6ull * m, "nodes"));
+  TRY(ensure(c, c->leaf_node, m, "leaves"));
+  TRY(ensure(c, c->leaf_first, m, "leaves"));
+  if (p.walk) {
+    TRY(ensure(c, c->mark, n, "targets"));
+    TRY(ensure(c, c->targets, n, "targets"));
+    TRY(ensure(c, c->gmark, n, "groups"));
+    TRY(ensure(c, c->groups, n, "groups"));
+    TRY(ensure(c, c->unit_n, m, "walk units"));
+    TRY(ensure(c, c->gacc, 3ull * n, "group sums"));
+    TRY(ensure(c, c->defer, uint64_t(p.slots) * kDefer, "deferred nodes"));
+    if (p.deep) TRY(ensure(c, c->deep_frames, uint64_t(p.slots) * (kLevels - kShallowFrames) * kWave, "walk stack"));
+  }
+  Scratch& s = *b;
+  s = Scratch{};
+  s.keys_a = c->keys_a.p;
+  s.keys_b = c->keys_b.p;
+  s.vals_a = c->vals_a.p;
+  s.vals_b = c->vals_b.p;
+  s.hist = c->hist.p;
+  s.dbase = c->dbase.p;
+  s.lookback = c->lookback.p;
+  s.lohi = c->lohi.p;
+  s.partial = c->partial.p;
+  s.misc = c->misc.p;
+  s.totals = c->totals.p;
+  s.posm = c->posm.p;
+  s.L = c->L.p;
+  s.H = c->H.p;
+  s.tcounts = c->tcounts.p;
+  s.tprefix = c->tprefix.p;
+  s.sid = c->sid.p;
+  s.tables = c->tables.p;
+  TreeDev& t = s.t;
   t.begin = c->begin.p;
   t.count = c->count.p;
   t.depth = c->depth.p;
@@ -192,286 +475,296 @@ TreeDev tree_view(Ctx* c) {
   t.leaf_node = c->leaf_node.p;
   t.leaf_first = c->leaf_first.p;
   t.level_off = c->level_off.p;
-  return t;
-}
-
-// The batch's system table: per-system parameters, chunks of <= kChunk
-// particles for the per-system bounds, and the system-id digit counts of the
-// sort's last passes. Staged in pinned memory, uploaded with one copy.
-int prepare_systems(Ctx* c, const HostSys* sys, uint32_t S, uint64_t* n_total) {
-  if (S == 0 || S > kMaxSystems) {
-    set_err(c, "a batch holds 1..%u systems, %u given", kMaxSystems, S);
-    return BH_ERR_INVALID;
-  }
-  // Empty systems take no part: the device table lists the others, in
-  // order, and a particle's system id indexes that list.
-  uint64_t n = 0, chunks = 0;
-  uint32_t mlx = 0, live = 0;
-  bool unsoftened = false;
-  for (uint32_t k = 0; k < S; ++k) {
-    TRY(check_args(c, sys[k].n, &sys[k].fp, sys[k].max_leaf, sys[k].group_size));
-    if (sys[k].n == 0) continue;
-    ++live;
-    n += sys[k].n;
-    chunks += (sys[k].n + kChunk - 1) / kChunk;
-    mlx = std::max(mlx, sys[k].max_leaf);
-    unsoftened |= static_cast<float>(sys[k].fp.eps2) == 0.0f;  // zero in the FP32 walk too
-  }
-  if (n > kMaxN) {
-    set_err(c, "batch of %llu particles exceeds the supported %llu", (unsigned long long)n, (unsigned long long)kMaxN);
-    return BH_ERR_TOO_LARGE;
-  }
-  *n_total = n;
-  if (n == 0) return BH_OK;
-  const uint32_t S_all = S;
-  S = live;
-  const size_t off_chunks = sizeof(SysDev) * S;
-  const size_t off_first = off_chunks + sizeof(uint4) * chunks;
-  const size_t off_hist = (off_first + 4 * (S + 1) + 15) / 16 * 16;
-  const size_t bytes = off_hist + 2 * 256 * 4;
-  if (c->host_tables_cap < bytes) {
-    if (c->host_tables) (void)hipHostFree(c->host_tables);
-    c->host_tables = nullptr;
-    c->host_tables_cap = 0;
-    const size_t want = bytes + bytes / 2;
-    if (hipHostMalloc(reinterpret_cast<void**>(&c->host_tables), want, hipHostMallocDefault) != hipSuccess) {
-      (void)hipGetLastError();
-      set_err(c, "pinned host allocation of %zu bytes failed", want);
-      return BH_ERR_OOM;
-    }
-    c->host_tables_cap = want;
-  }
-  std::memset(c->host_tables, 0, bytes);
-  SysDev* sd = reinterpret_cast<SysDev*>(c->host_tables);
-  uint4* ch = reinterpret_cast<uint4*>(c->host_tables + off_chunks);
-  uint32_t* first = reinterpret_cast<uint32_t*>(c->host_tables + off_first);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(c->host_tables + off_hist);
-  uint64_t off = 0, ci = 0;
-  for (uint32_t k = 0, src = 0; k < S; ++k, ++src) {
-    while (sys[src].n == 0) ++src;
-    const HostSys& h = sys[src];
-    sd[k].theta2 = h.fp.theta * h.fp.theta;
-    sd[k].eps2 = h.fp.eps2;
-    sd[k].G = h.fp.G;
-    sd[k].kick_dt = h.kick_dt;
-    sd[k].drift_dt = h.drift_dt;
-    sd[k].off = static_cast<uint32_t>(off);
-    sd[k].n = static_cast<uint32_t>(h.n);
-    sd[k].max_leaf = h.max_leaf;
-    sd[k].tsm = std::min(h.target_size, h.group_size);
-    sd[k].use_quad = h.fp.use_quadrupole != 0;
-    first[k] = static_cast<uint32_t>(ci);
-    for (uint64_t b = 0; b < h.n; b += kChunk)
-      ch[ci++] = make_uint4(k, static_cast<uint32_t>(off + b), static_cast<uint32_t>(off + std::min<uint64_t>(h.n, b + kChunk)), 0);
-    hist[k & 255] += static_cast<uint32_t>(h.n);
-    hist[256 + (k >> 8)] += static_cast<uint32_t>(h.n);
-    off += h.n;
-  }
-  first[S] = static_cast<uint32_t>(ci);
-  TRY(ensure(c, c->tables, bytes, "system table"));
-  TRY(ensure(c, c->sid, n, "system ids"));
-  HIP_OK(c, hipMemcpyAsync(c->tables.p, c->host_tables, bytes, hipMemcpyHostToDevice, c->stream));
-  c->n_sys = S;
-  c->n_chunks = static_cast<uint32_t>(chunks);
-  c->mlx = mlx;
-  c->sid_passes = S == 1 ? 0 : (S <= 256 ? 1 : 2);
-  c->any_unsoftened = unsoftened;
-  c->off_chunks = off_chunks;
-  c->off_chunk_first = off_first;
-  c->off_sid_hist = off_hist;
-  // single-system queries (export, accel) read the first system's shape
-  for (uint32_t k = 0; k < S_all; ++k) {
-    if (sys[k].n == 0) continue;
-    c->max_leaf = sys[k].max_leaf;
-    c->group_size = sys[k].group_size;
-    break;
+  t.misc = c->misc.p;
+  t.cap = m;
+  if (p.walk) {
+    s.mark = c->mark.p;
+    s.targets = c->targets.p;
+    s.gmark = c->gmark.p;
+    s.groups = c->groups.p;
+    s.unit_n = c->unit_n.p;
+    s.gacc = c->gacc.p;
+    s.defer = c->defer.p;
+    s.deep_frames = p.deep ? c->deep_frames.p : nullptr;
   }
   return BH_OK;
 }
 
-const SysDev* sys_dev(Ctx* c) { return reinterpret_cast<const SysDev*>(c->tables.p); }
-
-// build_flat_tree (flat_build_ttg.h:276-281) of every system of the prepared
-// batch, d_in[0..n) into the context; the Morton-ordered particles (each
-// system in its own range) go to d_out.
-int build(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint32_t n) {
-  hipStream_t s = c->stream;
-  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
-  TRY(ensure(c, c->keys_a, n, "keys"));
-  TRY(ensure(c, c->keys_b, n, "keys"));
-  TRY(ensure(c, c->vals_a, n, "indices"));
-  TRY(ensure(c, c->vals_b, n, "indices"));
-  TRY(ensure(c, c->hist, 10 * 256, "histograms"));
-  TRY(ensure(c, c->dbase, 10 * 256, "digit offsets"));
-  TRY(ensure(c, c->lookback, radix_lookback_words(n), "look-back"));
-  TRY(ensure(c, c->lohi, 6ull * c->n_sys, "bounds"));
-  TRY(ensure(c, c->partial, 6ull * c->n_chunks, "bounds"));
-  TRY(ensure(c, c->misc, 64, "flags"));
-  TRY(ensure(c, c->posm, n, "walk copy"));
-  TRY(ensure(c, c->L, n, "tree build"));
-  TRY(ensure(c, c->H, n, "tree build"));
-  TRY(ensure(c, c->tcounts, uint64_t(tiles) * kTreeCols, "tree build"));
-  TRY(ensure(c, c->tprefix, uint64_t(tiles) * kTreeCols, "tree build"));
-  TRY(ensure(c, c->level_off, kLevels + 2, "tree levels"));
-
-  const uint4* chunks = reinterpret_cast<const uint4*>(c->tables.p + c->off_chunks);
-  const uint32_t* chunk_first = reinterpret_cast<const uint32_t*>(c->tables.p + c->off_chunk_first);
-  HIP_OK(c, launch_fill_u32(c->misc.p, 0u, 64, s));
-  HIP_OK(c, launch_fill_u32(c->hist.p, 0u, 8 * 256, s));
-  if (c->sid_passes)
-    HIP_OK(c, hipMemcpyAsync(c->hist.p + 8 * 256, c->host_tables + c->off_sid_hist, 2 * 256 * 4, hipMemcpyHostToDevice, s));
-  HIP_OK(c, launch_fill_u32(c->lookback.p, 0u, radix_lookback_words(n), s));
-  HIP_OK(c, launch_fill_sid(chunks, c->n_chunks, c->sid.p, s));
-  HIP_OK(c, launch_bounds(d_in, chunks, c->n_chunks, chunk_first, c->n_sys, c->partial.p, c->misc.p, c->lohi.p, s));
-  HIP_OK(c, launch_keys_hist(d_in, n, c->sid.p, c->lohi.p, c->keys_a.p, c->hist.p, s));
-  const uint64_t lb_words = radix_lookback_words(n);
-  uint64_t* keys_sorted = nullptr;
+struct Sorted {
+  uint64_t* keys = nullptr;
   uint32_t* perm = nullptr;
-  HIP_OK(c, launch_radix_sort(c->keys_a.p, c->vals_a.p, c->keys_b.p, c->vals_b.p, n, c->sid_passes, c->sid.p, c->hist.p,
-                              c->dbase.p, c->lookback.p, c->lookback.p + (lb_words - 10), &keys_sorted, &perm, s));
-  HIP_OK(c, launch_gather(d_in, d_out, perm, c->posm.p, n, s));
-  HIP_OK(c, hipEventRecord(c->ev[1], s));
-  HIP_OK(c, launch_tree_count(keys_sorted, n, c->sid.p, sys_dev(c), c->mlx, c->L.p, c->H.p, c->tcounts.p, s));
-  HIP_OK(c, launch_tree_scan(c->tcounts.p, tiles, c->tprefix.p, c->misc.p + 8, s));
-  HIP_OK(c, hipMemcpyAsync(c->host_misc, c->misc.p, 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_OK(c, hipStreamSynchronize(s));
-  if (c->host_misc[0] != 0) {
-    set_err(c, "non-finite position, velocity or mass in the input");
-    return BH_ERR_NONFINITE;
-  }
-  uint64_t nodes = 0;
-  c->max_depth = 0;
-  for (int d = 0; d < kLevels; ++d) {
-    nodes += c->host_misc[8 + d];
-    if (c->host_misc[8 + d]) c->max_depth = static_cast<uint32_t>(d);
-  }
-  const uint32_t leaves = c->host_misc[8 + kLevels];
-  if (nodes >= (1ull << 29)) {
-    set_err(c, "tree of %llu nodes exceeds the supported 2^29", (unsigned long long)nodes);
-    return BH_ERR_TOO_LARGE;
-  }
-  const uint32_t m = static_cast<uint32_t>(nodes);
-  TRY(ensure(c, c->begin, m, "nodes"));
-  TRY(ensure(c, c->count, m, "nodes"));
-  TRY(ensure(c, c->cs, m, "nodes"));
-  TRY(ensure(c, c->first_child, m, "nodes"));
-  TRY(ensure(c, c->parent, m, "nodes"));
-  TRY(ensure(c, c->leaf_idx, m, "nodes"));
-  TRY(ensure(c, c->depth, m, "nodes"));
-  TRY(ensure(c, c->leaf, m, "nodes"));
-  TRY(ensure(c, c->nchild, m, "nodes"));
-  TRY(ensure(c, c->ctr, m, "nodes"));
-  TRY(ensure(c, c->rec, m, "nodes"));
-  TRY(ensure(c, c->mass, m, "nodes"));
-  TRY(ensure(c, c->com, 3ull * m, "nodes"));
-  TRY(ensure(c, c->iab, 6ull * m, "nodes"));
-  TRY(ensure(c, c->q64, 8ull * m, "nodes"));
-  TRY(ensure(c, c->nbox, 6ull * m, "nodes"));
+};
 
-  TRY(ensure(c, c->leaf_node, leaves, "leaves"));
-  TRY(ensure(c, c->leaf_first, leaves, "leaves"));
-  const TreeDev t = tree_view(c);
-  HIP_OK(c, launch_tree_emit(c->L.p, c->H.p, n, c->tprefix.p, c->misc.p + 8, t, s));
-  uint32_t level_nodes[kLevels];
-  for (int d = 0; d < kLevels; ++d) level_nodes[d] = c->host_misc[8 + d];
-  HIP_OK(c, launch_tree_finish(keys_sorted, c->posm.p, n, c->sid.p, c->max_depth, c->lohi.p, m, leaves, level_nodes, t, s));
-  HIP_OK(c, hipEventRecord(c->ev[2], s));
-  c->n = n;
-  c->n_nodes = m;
-  c->n_leaves = leaves;
-  c->sorted = d_out;
-  c->keys_sorted = keys_sorted;
-  c->perm_sorted = perm;
-  c->has_tree = true;
+// build_flat_tree (flat_build_ttg.h:276-281) of every system of the plan,
+// d_in[0..n) into the scratch; the Morton-ordered particles (each system in
+// its own range) go to d_out. `events`: record the phase events.
+int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_in, bh_particle* d_out, bool events,
+                  Sorted* out) {
+  hipStream_t s = c->stream;
+  const uint32_t n = p.n;
+  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
+  const uint4* chunks = reinterpret_cast<const uint4*>(b.tables + p.off_chunks);
+  const uint32_t* chunk_first = reinterpret_cast<const uint32_t*>(b.tables + p.off_first);
+  const SysDev* sys = reinterpret_cast<const SysDev*>(b.tables);
+  const uint64_t lb_words = radix_lookback_words(n);
+  HIP_OK(c, hipMemcpyAsync(b.tables, c->host_tables, p.table_bytes, hipMemcpyHostToDevice, s));
+  HIP_OK(c, launch_fill_u32(b.misc, 0u, kMiscWords, s));
+  HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(b.totals), 0u, 4, s));
+  HIP_OK(c, launch_fill_u32(b.hist, 0u, 8 * 256, s));
+  if (p.sid_passes)
+    HIP_OK(c, hipMemcpyAsync(b.hist + 8 * 256, c->host_tables + p.off_hist, 2 * 256 * 4, hipMemcpyHostToDevice, s));
+  HIP_OK(c, launch_fill_u32(b.lookback, 0u, lb_words, s));
+  HIP_OK(c, launch_fill_sid(chunks, p.n_chunks, b.sid, s));
+  HIP_OK(c, launch_bounds(d_in, chunks, p.n_chunks, chunk_first, p.S, b.partial, b.misc + kMiscNonfinite, b.lohi, s));
+  HIP_OK(c, launch_keys_hist(d_in, n, b.sid, b.lohi, b.keys_a, b.hist, s));
+  HIP_OK(c, launch_radix_sort(b.keys_a, b.vals_a, b.keys_b, b.vals_b, n, p.sid_passes, b.sid, b.hist, b.dbase,
+                              b.lookback, b.lookback + (lb_words - 10), &out->keys, &out->perm, s));
+  HIP_OK(c, launch_gather(d_in, d_out, out->perm, b.posm, n, s));
+  if (events) HIP_OK(c, hipEventRecord(c->ev[1], s));
+  HIP_OK(c, launch_tree_count(out->keys, n, b.sid, sys, p.mlx, b.L, b.H, b.tcounts, s));
+  HIP_OK(c, launch_tree_scan(b.tcounts, tiles, b.tprefix, b.misc + kMiscLevels, s));
+  HIP_OK(c, launch_tree_emit(b.L, b.H, n, b.tprefix, b.t, p.depth_plan, s));
+  HIP_OK(c, launch_tree_finish(out->keys, b.posm, n, b.sid, p.S, b.lohi, p.depth_plan, b.t, s));
+  if (events) HIP_OK(c, hipEventRecord(c->ev[2], s));
   return BH_OK;
 }
 
-// run_grouped_pass (grouped_ttg.h:297-323) over every system of the
-// context's tree, with each system's parameters from the system table.
-int walk(Ctx* c, bh_particle* parts, double* d_acc, uint32_t* d_npp, uint32_t* d_npc) {
+// run_grouped_pass (grouped_ttg.h:297-323) over every system of the tree in
+// the scratch, with each system's parameters from the system table.
+int enqueue_walk(Ctx* c, const Plan& p, const Scratch& b, bh_particle* parts, double* d_acc, uint32_t* d_npp,
+                 uint32_t* d_npc, bool events) {
   hipStream_t s = c->stream;
-  const uint32_t n = c->n;
-  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
-  TRY(ensure(c, c->mark, n, "targets"));
-  TRY(ensure(c, c->targets, n, "targets"));
-  TRY(ensure(c, c->gmark, n, "groups"));
-  TRY(ensure(c, c->groups, n, "groups"));
-  TRY(ensure(c, c->gacc, 3ull * n, "group sums"));
-  TRY(ensure(c, c->totals, 2, "totals"));
-  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), false, c->mark.p, c->tcounts.p,
-                           c->tprefix.p, c->targets.p, c->misc.p + 40, nullptr, s));
-  // per-target mode: the walk groups are the targets themselves
+  const uint32_t n = p.n;
+  const SysDev* sys = reinterpret_cast<const SysDev*>(b.tables);
   const bool grouped = c->walk_mode == BH_WALK_GROUPED;
-  if (grouped) TRY(ensure(c, c->unit_n, c->n_nodes, "walk units"));
-  HIP_OK(c, launch_targets(tree_view(c), c->n_nodes, n, c->sid.p, sys_dev(c), grouped, c->gmark.p, c->tcounts.p,
-                           c->tprefix.p, c->groups.p, c->misc.p + 41, grouped ? c->unit_n.p : nullptr, s));
-  HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(c->totals.p), 0u, 4, s));
-  HIP_OK(c, hipEventRecord(c->ev[3], s));
-  HIP_OK(c, hipMemcpyAsync(c->host_misc + 40, c->misc.p + 40, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_OK(c, hipStreamSynchronize(s));
-  (void)tiles;
-  const uint32_t n_groups = c->host_misc[41];
-  TRY(ensure(c, c->defer, uint64_t(n_groups) * kDefer, "deferred nodes"));
+  // a walk on its own (bh_accel) clears what an earlier walk left
+  HIP_OK(c, launch_fill_u32(b.misc + kMiscIncomplete, 0u, 1, s));
+  HIP_OK(c, launch_fill_u32(b.misc + kMiscQueue, 0u, 8, s));
+  HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(b.totals), 0u, 4, s));
+  HIP_OK(c, launch_targets(b.t, n, b.sid, sys, false, b.mark, b.tcounts, b.tprefix, b.targets, b.misc + kMiscTargets,
+                           nullptr, s));
+  // per-target mode: the walk units are the targets themselves
+  HIP_OK(c, launch_targets(b.t, n, b.sid, sys, grouped, b.gmark, b.tcounts, b.tprefix, b.groups, b.misc + kMiscGroups,
+                           grouped ? b.unit_n : nullptr, s));
+  if (events) HIP_OK(c, hipEventRecord(c->ev[3], s));
   WalkArgs a;
-  a.rec = c->rec.p;
-  a.q64 = c->q64.p;
-  a.ctr = c->ctr.p;
-  a.begin = c->begin.p;
-  a.count = c->count.p;
-  a.posm = c->posm.p;
-  a.groups = c->groups.p;
-  a.unit_n = grouped ? c->unit_n.p : nullptr;
-  a.tmark = c->mark.p;
-  a.sid = c->sid.p;
-  a.sys = sys_dev(c);
-  a.gacc = c->gacc.p;
-  a.nbox = c->nbox.p;
-  a.defer = c->defer.p;
-  a.n_groups = n_groups;
-  a.n_nodes = c->n_nodes;
-  a.max_depth = c->max_depth;
-  a.deep_frames = nullptr;
-  if (c->max_depth + 1 > kShallowFrames) {
-    TRY(ensure(c, c->deep_frames, uint64_t(n_groups) * (kLevels - kShallowFrames) * kWave, "walk stack"));
-    a.deep_frames = c->deep_frames.p;
-  }
-  a.n_parts = c->n;
+  a.rec = b.t.rec;
+  a.q64 = b.t.q64;
+  a.ctr = b.t.ctr;
+  a.begin = b.t.begin;
+  a.count = b.t.count;
+  a.posm = b.posm;
+  a.groups = b.groups;
+  a.unit_n = grouped ? b.unit_n : nullptr;
+  a.tmark = b.mark;
+  a.sid = b.sid;
+  a.sys = sys;
+  a.gacc = b.gacc;
+  a.nbox = b.t.nbox;
+  a.defer = b.defer;
+  a.deep_frames = p.deep ? b.deep_frames : nullptr;
+  a.misc = b.misc;
+  a.n_nodes = p.cap_nodes;
+  a.n_parts = n;
   a.parts = parts;
   a.acc = d_acc;
   a.n_pp = d_npp;
   a.n_pc = d_npc;
-  a.totals = c->totals.p;
-  HIP_OK(c, hipEventRecord(c->ev[4], s));
-  HIP_OK(c, launch_walk(a, c->precision == BH_FP64, c->any_unsoftened, s));
-  HIP_OK(c, hipEventRecord(c->ev[5], s));
-  c->have_stats = true;
+  a.totals = b.totals;
+  if (events) HIP_OK(c, hipEventRecord(c->ev[4], s));
+  HIP_OK(c, launch_walk(a, p.slots, p.deep, c->precision == BH_FP64, p.unsoftened, s));
+  if (events) HIP_OK(c, hipEventRecord(c->ev[5], s));
+  return BH_OK;
+}
+
+// Copies the flag/count words and the totals to the pinned report and
+// waits for the stream: the step's only synchronisation.
+int finish(Ctx* c, const Scratch& b) {
+  HIP_OK(c, hipMemcpyAsync(c->host_misc, b.misc, kMiscWords * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipMemcpyAsync(c->host_misc + kMiscWords, b.totals, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+GraphKey graph_key(const Ctx* c, const Plan& p, const void* in, const void* out) {
+  GraphKey k;
+  std::memset(&k, 0, sizeof k);
+  k.in = in;
+  k.out = out;
+  k.gen = c->gen;
+  k.n = p.n;
+  k.S = p.S;
+  k.n_chunks = p.n_chunks;
+  k.mlx = p.mlx;
+  k.cap_nodes = p.cap_nodes;
+  k.depth_plan = p.depth_plan;
+  k.slots = p.slots;
+  k.table_bytes = static_cast<uint32_t>(p.table_bytes);
+  k.deep = p.deep;
+  k.unsoftened = p.unsoftened;
+  k.precision = static_cast<uint8_t>(c->precision);
+  k.mode = static_cast<uint8_t>(c->walk_mode);
+  return k;
+}
+
+// The step as one graph launch: replays the graph captured for the same
+// launches, or captures and instantiates it first (the least recently used
+// of kGraphCache graphs makes room).
+int launch_graph(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_in, bh_particle* d_out, Sorted* sorted) {
+  const GraphKey key = graph_key(c, p, d_in, d_out);
+  GraphEntry* hit = nullptr;
+  for (auto& g : c->graphs)
+    if (g.key == key) hit = &g;
+  // the sort's output buffers follow from the pass count alone
+  const int passes = 8 + p.sid_passes;
+  sorted->keys = (passes % 2 == 0) ? b.keys_a : b.keys_b;
+  sorted->perm = (passes % 2 == 0) ? b.vals_a : b.vals_b;
+  if (!hit) {
+    HIP_OK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    Sorted sd;
+    int rc = enqueue_build(c, p, b, d_in, d_out, false, &sd);
+    if (rc == BH_OK) rc = enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (rc != BH_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIP_OK(c, ec);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_OK(c, ei);
+    if (sd.keys != sorted->keys || sd.perm != sorted->perm) {
+      (void)hipGraphExecDestroy(exec);
+      set_err(c, "radix sort output buffers differ from the pass-count rule");
+      return BH_ERR_HIP;
+    }
+    if (c->graphs.size() >= kGraphCache) {
+      auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                  [](const GraphEntry& x, const GraphEntry& y) { return x.used < y.used; });
+      (void)hipGraphExecDestroy(lru->exec);
+      c->graphs.erase(lru);
+    }
+    c->graphs.push_back(GraphEntry{key, exec, 0});
+    hit = &c->graphs.back();
+  }
+  hit->used = ++c->graph_clock;
+  HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
+  HIP_OK(c, hipGraphLaunch(hit->exec, c->stream));
+  HIP_OK(c, hipEventRecord(c->ev[5], c->stream));
   return BH_OK;
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
-  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.f;
+  }
   return ms;
+}
+
+// Keeps the report of a finished step or build.
+void take_report(Ctx* c, const Plan& p, const Scratch& b, const Sorted& sd, bh_particle* d_out, bool walked) {
+  const uint32_t* m = c->host_misc;
+  uint64_t nodes = 0;
+  for (int d = 0; d < kLevels; ++d) nodes += m[kMiscLevels + d];
+  c->n = p.n;
+  c->n_nodes = static_cast<uint32_t>(nodes);
+  c->n_leaves = m[kMiscLevels + kLevels];
+  c->max_depth = m[kMiscMaxDepth];
+  if (walked) {
+    c->n_targets = m[kMiscTargets];
+    c->n_groups = m[kMiscGroups];
+    std::memcpy(c->last_totals, m + kMiscWords, sizeof c->last_totals);
+  }
+  c->plan = p;
+  c->sc = b;
+  c->sorted = d_out;
+  c->keys_sorted = sd.keys;
+  c->perm_sorted = sd.perm;
+}
+
+// A step (walk = true) or tree build of the systems planned in the pinned
+// table, on the context's buffers: eagerly or as a graph, then the report.
+// A tree that outgrew the plan is built again with room for it.
+int run(Ctx* c, Plan p, const bh_particle* d_in, bh_particle* d_out, bool walk, bh_step_timing* timing) {
+  c->have_stats = false;
+  c->has_tree = false;
+  for (int attempt = 0;; ++attempt) {
+    c->reruns = static_cast<uint32_t>(attempt);
+    plan_shape(c, &p, walk);
+    Scratch b;
+    TRY(provision(c, p, &b));
+    Sorted sd;
+    const bool graph = walk && c->graph_mode == BH_GRAPH_ON;
+    if (graph) {
+      TRY(launch_graph(c, p, b, d_in, d_out, &sd));
+    } else {
+      HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
+      TRY(enqueue_build(c, p, b, d_in, d_out, true, &sd));
+      if (walk) TRY(enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, true));
+    }
+    TRY(finish(c, b));
+    const uint32_t* m = c->host_misc;
+    if (m[kMiscNonfinite]) {
+      set_err(c, "non-finite position, velocity or mass in the input");
+      return BH_ERR_NONFINITE;
+    }
+    const uint32_t abort = m[kMiscAbort];
+    if (abort) {
+      if (attempt >= 3) {
+        set_err(c, "the tree outgrew its planned arrays on every attempt (abort word %u)", abort);
+        return BH_ERR_HIP;
+      }
+      if (abort & kAbortNodes) {
+        uint64_t need = m[kMiscLevels + kLevels];
+        uint64_t nodes = 0;
+        for (int d = 0; d < kLevels; ++d) nodes += m[kMiscLevels + d];
+        need = std::max(need, nodes);
+        c->cap_nodes = static_cast<uint32_t>(std::min<uint64_t>(need + need / 8 + 64, 1ull << 30));
+      }
+      if (abort & kAbortDepth) {
+        c->depth_hint = std::max(c->depth_known ? c->depth_hint : 0u, m[kMiscMaxDepth]);
+        c->depth_known = true;
+      }
+      continue;
+    }
+    // later steps plan for the deepest tree seen
+    c->depth_hint = c->depth_known ? std::max(c->depth_hint, m[kMiscMaxDepth]) : m[kMiscMaxDepth];
+    c->depth_known = true;
+    take_report(c, p, b, sd, d_out, walk);
+    c->has_tree = true;
+    c->have_stats = walk;
+    c->last_graph = graph;
+    if (timing) {
+      if (graph) {
+        timing->tree_build_ms = 0.0;
+        timing->traversal_ms = elapsed(c->ev[0], c->ev[5]);  // the whole step: a graph has no inner events
+      } else {
+        timing->tree_build_ms = elapsed(c->ev[0], c->ev[2]);
+        timing->traversal_ms = walk ? elapsed(c->ev[2], c->ev[5]) : 0.0;
+      }
+      timing->exchange_ms = 0.0;
+    }
+    if (walk && m[kMiscIncomplete]) {
+      set_err(c, "incomplete tree: the walk met an interior node it could not open");
+      return BH_ERR_INCOMPLETE;
+    }
+    return BH_OK;
+  }
 }
 
 // One step of a batch: systems sys[0..S) own consecutive ranges of d_in.
 int step_batch(Ctx* c, const bh_particle* d_in, bh_particle* d_out, const HostSys* sys, uint32_t S, bh_step_timing* timing) {
   if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
-  uint64_t n = 0;
   HIP_OK(c, hipSetDevice(c->device));
-  TRY(prepare_systems(c, sys, S, &n));
-  if (n == 0) return BH_OK;
-  c->have_stats = false;
-  HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
-  TRY(build(c, d_in, d_out, static_cast<uint32_t>(n)));
-  TRY(walk(c, d_out, nullptr, nullptr, nullptr));
-  HIP_OK(c, hipStreamSynchronize(c->stream));
-  if (timing) {
-    timing->tree_build_ms = elapsed(c->ev[0], c->ev[2]);
-    timing->traversal_ms = elapsed(c->ev[2], c->ev[5]);
-    timing->exchange_ms = 0.0;
-  }
-  return BH_OK;
+  Plan p;
+  TRY(plan_systems(c, sys, S, false, &p));
+  if (p.n == 0) return BH_OK;
+  return run(c, p, d_in, d_out, true, timing);
 }
 
 int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
@@ -482,17 +775,20 @@ int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n,
     if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
     return BH_OK;
   }
-  const HostSys one{n, *fp, max_leaf, group_size, target_size, kick_dt, drift_dt};
+  const HostSys one{n, *fp, max_leaf, group_size, target_size, kick_dt, drift_dt, 0};
   return step_batch(c, d_in, d_out, &one, 1, timing);
 }
 
 std::vector<HostSys> host_systems(const bh_system* systems, uint32_t S) {
   std::vector<HostSys> v(S);
   for (uint32_t k = 0; k < S; ++k)
-    v[k] = HostSys{systems[k].n, systems[k].fp, systems[k].max_leaf, systems[k].group_size, systems[k].target_size,
-                   systems[k].kick_dt, systems[k].drift_dt};
+    v[k] = HostSys{systems[k].n,         systems[k].fp,      systems[k].max_leaf,
+                   systems[k].group_size, systems[k].target_size, systems[k].kick_dt,
+                   systems[k].drift_dt, systems[k].walk_tile};
   return v;
 }
+
+bool overlap(const bh_particle* a, const bh_particle* b, uint64_t n) { return n && a + n > b && b + n > a; }
 
 Ctx* as(bh_ctx* c) { return reinterpret_cast<Ctx*>(c); }
 
@@ -566,14 +862,28 @@ int bh_ctx_create(bh_ctx** out, int device) {
     delete c;
     return BH_ERR_HIP;
   }
+  // pinned staging sized so that small tables never have to grow
+  const size_t table0 = 64 << 10;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&c->host_misc), 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&c->host_misc), (kMiscWords + 4) * sizeof(uint32_t), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->host_tables), table0, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
+    if (c->host_tables) (void)hipHostFree(c->host_tables);
+    if (c->host_misc) (void)hipHostFree(c->host_misc);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BH_ERR_HIP;
   }
+  c->host_tables_cap = table0;
   for (auto& e : c->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      for (auto& f : c->ev)
+        if (f) (void)hipEventDestroy(f);
+      (void)hipHostFree(c->host_tables);
+      (void)hipHostFree(c->host_misc);
+      (void)hipStreamDestroy(c->stream);
       delete c;
       return BH_ERR_HIP;
     }
@@ -587,9 +897,11 @@ void bh_ctx_destroy(bh_ctx* h) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   release(c->pa);
   release(c->pb);
   release(c->tmp_in);
+  release(c->tmp_out);
   release(c->keys_a);
   release(c->keys_b);
   release(c->vals_a);
@@ -610,11 +922,11 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->targets);
   release(c->gmark);
   release(c->groups);
+  release(c->unit_n);
   release(c->defer);
   release(c->deep_frames);
   release(c->gacc);
   release(c->nbox);
-  release(c->unit_n);
   release(c->begin);
   release(c->count);
   release(c->cs);
@@ -633,13 +945,13 @@ void bh_ctx_destroy(bh_ctx* h) {
   release(c->com);
   release(c->iab);
   release(c->q64);
-
-  release(c->dsrc);
-  release(c->didx);
-  release(c->dpart);
-  release(c->dacc);
   release(c->tables);
   release(c->sid);
+  release(c->dsrc);
+  release(c->didx);
+  release(c->dcnt);
+  release(c->dpart);
+  release(c->dacc);
   if (c->host_tables) (void)hipHostFree(c->host_tables);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -675,12 +987,45 @@ int bh_set_walk_mode(bh_ctx* h, int mode) {
   return BH_OK;
 }
 
+int bh_set_graph_mode(bh_ctx* h, int mode) {
+  Ctx* c = as(h);
+  if (!c) return BH_ERR_INVALID;
+  if (mode != BH_GRAPH_OFF && mode != BH_GRAPH_ON) {
+    set_err(c, "unknown graph mode %d", mode);
+    return BH_ERR_INVALID;
+  }
+  if (mode == BH_GRAPH_OFF) {
+    (void)hipSetDevice(c->device);
+    drop_graphs(c);
+  }
+  c->graph_mode = mode;
+  return BH_OK;
+}
+
+int bh_set_walk_tile(bh_ctx* h, uint32_t tile) {
+  Ctx* c = as(h);
+  if (!c) return BH_ERR_INVALID;
+  if (tile < kMinTile || tile > kGroupCap) {
+    set_err(c, "walk tile %u outside %u..%u", tile, kMinTile, kGroupCap);
+    return BH_ERR_INVALID;
+  }
+  c->walk_tile = tile;
+  return BH_OK;
+}
+
+int bh_graph_count(bh_ctx* h, uint32_t* count) {
+  Ctx* c = as(h);
+  if (!c || !count) return BH_ERR_INVALID;
+  *count = static_cast<uint32_t>(c->graphs.size());
+  return BH_OK;
+}
+
 int bh_step_device(bh_ctx* h, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
                    uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
                    bh_step_timing* timing) {
   Ctx* c = as(h);
   if (!c || !fp || (n && (!d_in || !d_out))) return c ? (set_err(c, "null argument"), BH_ERR_INVALID) : BH_ERR_INVALID;
-  if (n && d_in + n > d_out && d_out + n > d_in) {
+  if (overlap(d_in, d_out, n)) {
     set_err(c, "d_in and d_out overlap");
     return BH_ERR_INVALID;
   }
@@ -698,12 +1043,11 @@ int bh_step(bh_ctx* h, bh_particle* particles, uint64_t n, const bh_force_params
   }
   HIP_OK(c, hipSetDevice(c->device));
   TRY(ensure(c, c->tmp_in, n, "particles"));
-  TRY(ensure(c, c->pb, n, "particles"));
+  TRY(ensure(c, c->tmp_out, n, "particles"));
   HIP_OK(c, hipMemcpyAsync(c->tmp_in.p, particles, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
-  TRY(step_device(c, c->tmp_in.p, c->pb.p, n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing));
-  HIP_OK(c, hipMemcpyAsync(particles, c->pb.p, n * sizeof(bh_particle), hipMemcpyDeviceToHost, c->stream));
+  TRY(step_device(c, c->tmp_in.p, c->tmp_out.p, n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing));
+  HIP_OK(c, hipMemcpyAsync(particles, c->tmp_out.p, n * sizeof(bh_particle), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
-  c->state_n = 0;  // pb no longer holds a resident state
   return BH_OK;
 }
 
@@ -714,7 +1058,7 @@ int bh_step_batch(bh_ctx* h, const bh_particle* d_in, bh_particle* d_out, const 
   const std::vector<HostSys> v = host_systems(systems, n_systems);
   uint64_t n = 0;
   for (const auto& q : v) n += q.n;
-  if (d_in + n > d_out && d_out + n > d_in) {
+  if (overlap(d_in, d_out, n)) {
     set_err(c, "d_in and d_out overlap");
     return BH_ERR_INVALID;
   }
@@ -734,12 +1078,11 @@ int bh_step_batch_host(bh_ctx* h, bh_particle* particles, const bh_system* syste
   }
   HIP_OK(c, hipSetDevice(c->device));
   TRY(ensure(c, c->tmp_in, n, "particles"));
-  TRY(ensure(c, c->pb, n, "particles"));
+  TRY(ensure(c, c->tmp_out, n, "particles"));
   if (n) HIP_OK(c, hipMemcpyAsync(c->tmp_in.p, particles, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
-  TRY(step_batch(c, c->tmp_in.p, c->pb.p, v.data(), n_systems, timing));
-  if (n) HIP_OK(c, hipMemcpyAsync(particles, c->pb.p, n * sizeof(bh_particle), hipMemcpyDeviceToHost, c->stream));
+  TRY(step_batch(c, c->tmp_in.p, c->tmp_out.p, v.data(), n_systems, timing));
+  if (n) HIP_OK(c, hipMemcpyAsync(particles, c->tmp_out.p, n * sizeof(bh_particle), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
-  c->state_n = 0;
   return BH_OK;
 }
 
@@ -764,6 +1107,10 @@ int bh_state_step(bh_ctx* h, const bh_force_params* fp, uint32_t max_leaf, uint3
                   double kick_dt, double drift_dt, bh_step_timing* timing) {
   Ctx* c = as(h);
   if (!c || !fp) return BH_ERR_INVALID;
+  if (c->state_n == 0) {
+    set_err(c, "no resident state: upload particles with bh_state_upload first");
+    return BH_ERR_INVALID;
+  }
   bh_particle* in = c->cur == 0 ? c->pa.p : c->pb.p;
   bh_particle* out = c->cur == 0 ? c->pb.p : c->pa.p;
   TRY(step_device(c, in, out, c->state_n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing));
@@ -774,6 +1121,10 @@ int bh_state_step(bh_ctx* h, const bh_force_params* fp, uint32_t max_leaf, uint3
 int bh_state_step_batch(bh_ctx* h, const bh_system* systems, uint32_t n_systems, bh_step_timing* timing) {
   Ctx* c = as(h);
   if (!c || !systems) return c ? (set_err(c, "null argument"), BH_ERR_INVALID) : BH_ERR_INVALID;
+  if (c->state_n == 0) {
+    set_err(c, "no resident state: upload particles with bh_state_upload first");
+    return BH_ERR_INVALID;
+  }
   const std::vector<HostSys> v = host_systems(systems, n_systems);
   uint64_t n = 0;
   for (const auto& q : v) n += q.n;
@@ -821,18 +1172,13 @@ int bh_build_tree(bh_ctx* h, const bh_particle* host, uint64_t n, uint32_t max_l
   }
   HIP_OK(c, hipSetDevice(c->device));
   TRY(ensure(c, c->tmp_in, n, "particles"));
-  TRY(ensure(c, c->pb, n, "particles"));
+  TRY(ensure(c, c->tmp_out, n, "particles"));
   HIP_OK(c, hipMemcpyAsync(c->tmp_in.p, host, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
-  c->have_stats = false;
   const bh_force_params dflt{0.5, 0.0, 1.0, 1, 0};
-  const HostSys one{n, dflt, max_leaf, group_size, group_size, 0.0, 0.0};
-  uint64_t total = 0;
-  TRY(prepare_systems(c, &one, 1, &total));
-  HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
-  TRY(build(c, c->tmp_in.p, c->pb.p, static_cast<uint32_t>(n)));
-  c->state_n = 0;
-  HIP_OK(c, hipStreamSynchronize(c->stream));
-  return BH_OK;
+  const HostSys one{n, dflt, max_leaf, group_size, group_size, 0.0, 0.0, 0};
+  Plan p;
+  TRY(plan_systems(c, &one, 1, false, &p));
+  return run(c, p, c->tmp_in.p, c->tmp_out.p, false, nullptr);
 }
 
 int bh_tree_sizes(bh_ctx* h, uint64_t* n_nodes, uint64_t* n_groups, double* dom4) {
@@ -843,14 +1189,15 @@ int bh_tree_sizes(bh_ctx* h, uint64_t* n_nodes, uint64_t* n_groups, double* dom4
     return BH_ERR_NO_TREE;
   }
   HIP_OK(c, hipSetDevice(c->device));
+  const TreeDev& t = c->sc.t;
   if (n_nodes) *n_nodes = c->n_nodes;
   if (n_groups) {
     const uint32_t m = c->n_nodes;
     std::vector<uint32_t> count(m), parent(m);
     std::vector<uint8_t> leaf(m);
-    HIP_OK(c, hipMemcpy(count.data(), c->count.p, m * 4, hipMemcpyDeviceToHost));
-    HIP_OK(c, hipMemcpy(parent.data(), c->parent.p, m * 4, hipMemcpyDeviceToHost));
-    HIP_OK(c, hipMemcpy(leaf.data(), c->leaf.p, m, hipMemcpyDeviceToHost));
+    HIP_OK(c, hipMemcpy(count.data(), t.count, m * 4ull, hipMemcpyDeviceToHost));
+    HIP_OK(c, hipMemcpy(parent.data(), t.parent, m * 4ull, hipMemcpyDeviceToHost));
+    HIP_OK(c, hipMemcpy(leaf.data(), t.leaf, m, hipMemcpyDeviceToHost));
     uint64_t g = 0;
     for (uint32_t u = 0; u < m; ++u) {
       const uint32_t pc = u == 0 ? std::numeric_limits<uint32_t>::max() : count[parent[u]];
@@ -860,7 +1207,7 @@ int bh_tree_sizes(bh_ctx* h, uint64_t* n_nodes, uint64_t* n_groups, double* dom4
   }
   if (dom4) {
     double4 r;
-    HIP_OK(c, hipMemcpy(&r, c->ctr.p, sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_OK(c, hipMemcpy(&r, t.ctr, sizeof(double4), hipMemcpyDeviceToHost));
     dom4[0] = r.x;
     dom4[1] = r.y;
     dom4[2] = r.z;
@@ -881,12 +1228,13 @@ int bh_tree_export(bh_ctx* h, int layout, uint64_t* keys, uint32_t* perm, bh_par
     set_err(c, "unknown layout %d", layout);
     return BH_ERR_INVALID;
   }
-  if (c->n_sys != 1) {
+  if (c->plan.S != 1) {
     set_err(c, "tree export covers single-system trees");
     return BH_ERR_INVALID;
   }
   HIP_OK(c, hipSetDevice(c->device));
   HIP_OK(c, hipStreamSynchronize(c->stream));
+  const TreeDev& t = c->sc.t;
   const uint32_t n = c->n, m = c->n_nodes;
   if (keys) HIP_OK(c, hipMemcpy(keys, c->keys_sorted, uint64_t(n) * 8, hipMemcpyDeviceToHost));
   if (perm) HIP_OK(c, hipMemcpy(perm, c->perm_sorted, uint64_t(n) * 4, hipMemcpyDeviceToHost));
@@ -895,17 +1243,17 @@ int bh_tree_export(bh_ctx* h, int layout, uint64_t* keys, uint32_t* perm, bh_par
   std::vector<uint8_t> depth(m), leaf(m), nchild(m);
   std::vector<double4> ctr(m);
   std::vector<double> mass(m), com(3ull * m), ib(6ull * m);
-  HIP_OK(c, hipMemcpy(begin.data(), c->begin.p, m * 4ull, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(count.data(), c->count.p, m * 4ull, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(first_child.data(), c->first_child.p, m * 4ull, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(parent.data(), c->parent.p, m * 4ull, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(depth.data(), c->depth.p, m, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(leaf.data(), c->leaf.p, m, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(nchild.data(), c->nchild.p, m, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(ctr.data(), c->ctr.p, m * sizeof(double4), hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(mass.data(), c->mass.p, m * 8ull, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(com.data(), c->com.p, m * 24ull, hipMemcpyDeviceToHost));
-  HIP_OK(c, hipMemcpy(ib.data(), c->iab.p, m * 48ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(begin.data(), t.begin, m * 4ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(count.data(), t.count, m * 4ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(first_child.data(), t.first_child, m * 4ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(parent.data(), t.parent, m * 4ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(depth.data(), t.depth, m, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(leaf.data(), t.leaf, m, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(nchild.data(), t.nchild, m, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(ctr.data(), t.ctr, m * sizeof(double4), hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(mass.data(), t.mass, m * 8ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(com.data(), t.com, m * 24ull, hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(ib.data(), t.iab, m * 48ull, hipMemcpyDeviceToHost));
   std::vector<uint32_t> ref_of, grp;
   if (layout == BH_LAYOUT_REFERENCE) {
     reference_order(count, leaf, first_child, nchild, c->group_size, ref_of, grp);
@@ -955,16 +1303,23 @@ int bh_accel(bh_ctx* h, const bh_force_params* fp, uint32_t target_size, double*
     return BH_ERR_NO_TREE;
   }
   TRY(check_args(c, c->n, fp, c->max_leaf, c->group_size));
-  if (c->n_sys != 1) {
+  if (c->plan.S != 1) {
     set_err(c, "bh_accel covers single-system trees");
     return BH_ERR_INVALID;
   }
   HIP_OK(c, hipSetDevice(c->device));
   const uint32_t n = c->n;
-  DevBuf<double>& da = c->dacc;
-  TRY(ensure(c, da, 3ull * n, "accelerations"));
-  DevBuf<uint32_t>& di = c->didx;
-  TRY(ensure(c, di, 2ull * n, "interaction counts"));
+  TRY(ensure(c, c->dacc, 3ull * n, "accelerations"));
+  TRY(ensure(c, c->dcnt, 2ull * n, "interaction counts"));
+  // the walk's buffers for the built tree (the build planned none)
+  Plan p = c->plan;
+  p.walk = true;
+  Scratch b;
+  TRY(provision(c, p, &b));
+  if (b.t.begin != c->sc.t.begin || b.keys_a != c->sc.keys_a || b.misc != c->sc.misc) {
+    set_err(c, "the tree's buffers moved since it was built");
+    return BH_ERR_NO_TREE;
+  }
   // the walk's parameters for the built tree: rewrite its system entry
   HIP_OK(c, hipStreamSynchronize(c->stream));
   SysDev* sd = reinterpret_cast<SysDev*>(c->host_tables);
@@ -975,18 +1330,31 @@ int bh_accel(bh_ctx* h, const bh_force_params* fp, uint32_t target_size, double*
   sd->drift_dt = 0.0;
   sd->tsm = std::min(target_size, c->group_size);
   sd->use_quad = fp->use_quadrupole != 0;
-  c->any_unsoftened = static_cast<float>(fp->eps2) == 0.0f;
-  HIP_OK(c, hipMemcpyAsync(c->tables.p, c->host_tables, sizeof(SysDev), hipMemcpyHostToDevice, c->stream));
-  TRY(walk(c, nullptr, da.p, di.p, di.p + n));
-  if (acc) HIP_OK(c, hipMemcpyAsync(acc, da.p, 24ull * n, hipMemcpyDeviceToHost, c->stream));
-  if (n_pp) HIP_OK(c, hipMemcpyAsync(n_pp, di.p, 4ull * n, hipMemcpyDeviceToHost, c->stream));
-  if (n_pc) HIP_OK(c, hipMemcpyAsync(n_pc, di.p + n, 4ull * n, hipMemcpyDeviceToHost, c->stream));
-  unsigned long long t2[2] = {0, 0};
-  HIP_OK(c, hipMemcpyAsync(t2, c->totals.p, sizeof t2, hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(c, hipStreamSynchronize(c->stream));
+  p.unsoftened = static_cast<float>(fp->eps2) == 0.0f;
+  HIP_OK(c, hipMemcpyAsync(b.tables, c->host_tables, sizeof(SysDev), hipMemcpyHostToDevice, c->stream));
+  c->have_stats = false;
+  c->reruns = 0;
+  HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
+  HIP_OK(c, hipEventRecord(c->ev[1], c->stream));
+  HIP_OK(c, hipEventRecord(c->ev[2], c->stream));
+  TRY(enqueue_walk(c, p, b, nullptr, c->dacc.p, c->dcnt.p, c->dcnt.p + n, true));
+  TRY(finish(c, b));
+  const uint32_t* m = c->host_misc;
+  c->n_targets = m[kMiscTargets];
+  c->n_groups = m[kMiscGroups];
+  std::memcpy(c->last_totals, m + kMiscWords, sizeof c->last_totals);
+  c->have_stats = true;
+  c->last_graph = false;
+  if (acc) HIP_OK(c, hipMemcpy(acc, c->dacc.p, 24ull * n, hipMemcpyDeviceToHost));
+  if (n_pp) HIP_OK(c, hipMemcpy(n_pp, c->dcnt.p, 4ull * n, hipMemcpyDeviceToHost));
+  if (n_pc) HIP_OK(c, hipMemcpy(n_pc, c->dcnt.p + n, 4ull * n, hipMemcpyDeviceToHost));
   if (totals) {
-    totals[0] = t2[0];
-    totals[1] = t2[1];
+    totals[0] = c->last_totals[0];
+    totals[1] = c->last_totals[1];
+  }
+  if (m[kMiscIncomplete]) {
+    set_err(c, "incomplete tree: the walk met an interior node it could not open");
+    return BH_ERR_INCOMPLETE;
   }
   return BH_OK;
 }
@@ -999,18 +1367,23 @@ int bh_last_step_stats(bh_ctx* h, bh_step_stats* st) {
     return BH_ERR_NO_TREE;
   }
   HIP_OK(c, hipSetDevice(c->device));
-  unsigned long long t2[2] = {0, 0};
-  HIP_OK(c, hipMemcpyAsync(t2, c->totals.p, sizeof t2, hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(c, hipStreamSynchronize(c->stream));
-  st->sort_ms = elapsed(c->ev[0], c->ev[1]);
-  st->tree_ms = elapsed(c->ev[1], c->ev[2]);
-  st->targets_ms = elapsed(c->ev[2], c->ev[3]);
-  st->walk_ms = elapsed(c->ev[4], c->ev[5]);
-  st->n_pp = t2[0];
-  st->n_pc = t2[1];
+  if (c->last_graph) {
+    st->sort_ms = st->tree_ms = st->targets_ms = st->walk_ms = 0.0;
+  } else {
+    st->sort_ms = elapsed(c->ev[0], c->ev[1]);
+    st->tree_ms = elapsed(c->ev[1], c->ev[2]);
+    st->targets_ms = elapsed(c->ev[2], c->ev[3]);
+    st->walk_ms = elapsed(c->ev[4], c->ev[5]);
+  }
+  st->step_ms = elapsed(c->ev[0], c->ev[5]);
+  st->n_pp = c->last_totals[0];
+  st->n_pc = c->last_totals[1];
   st->n_nodes = c->n_nodes;
   st->n_leaves = c->n_leaves;
-  st->n_targets = c->host_misc[40];
+  st->n_targets = c->n_targets;
+  st->n_units = c->n_groups;
+  st->graph = c->last_graph ? 1 : 0;
+  st->reruns = c->reruns;
   return BH_OK;
 }
 
@@ -1081,9 +1454,10 @@ int bh_energies(bh_ctx* h, const bh_particle* host, uint64_t n, double G, double
   std::vector<double> part(uint64_t(blocks) * splits);
   HIP_OK(c, hipMemcpyAsync(part.data(), c->dpart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
+  // the kernel counts every unordered pair once
   double sum = 0.0;
   for (double v : part) sum += v;
-  *potential = -0.5 * G * sum;
+  *potential = -G * sum;
   return BH_OK;
 }
 

@@ -35,16 +35,35 @@ struct SysDev {
   uint32_t max_leaf;
   uint32_t tsm;      // min(target_size, group_size)
   int32_t use_quad;
-  int32_t pad;
+  uint32_t tile;     // walk tile: particles a walk unit holds (kMinTile..kGroupCap)
 };
 static_assert(sizeof(SysDev) == 64, "SysDev layout");
 
 constexpr uint32_t kChunk = 65536;   // particles per bounds block
 constexpr uint32_t kMaxSystems = 65535;
 constexpr uint64_t kMaxN = (1ull << 30) - 1;  // particles per step (all systems)
-constexpr uint32_t kGroupCap = 128;  // walk groups: topmost nodes with leaf || count <= max(tsm, kGroupCap)
+constexpr uint32_t kGroupCap = 128;  // largest walk tile: a walk unit's particles fill at most two slots per lane
+constexpr uint32_t kMinTile = 16;    // smallest walk tile
 constexpr uint32_t kDefer = 512;     // deferred nodes a group may hand its targets
 constexpr uint32_t kShallowFrames = 12;  // walk stack of trees with at most this many levels
+constexpr uint32_t kWalkSlots = 4096;    // walk workgroups at most: 256 CUs x 4 SIMDs x 4 waves
+
+// The step's flag and count words (`misc`), cleared when a step starts and
+// read back once when it ends. Kernels after the tree's node count test the
+// abort word and return at once when it is set, so a step whose tree did not
+// fit the planned arrays or levels leaves its output untouched and is run
+// again with larger ones.
+constexpr int kMiscNonfinite = 0;   // some position, velocity or mass is NaN or infinite
+constexpr int kMiscAbort = 1;       // kAbortNodes | kAbortDepth
+constexpr int kMiscIncomplete = 2;  // the walk met an interior node it could not open
+constexpr int kMiscMaxDepth = 3;    // deepest level of the tree
+constexpr int kMiscLevels = 8;      // [8, 8 + kLevels): nodes per level; [8 + kLevels]: leaves
+constexpr int kMiscTargets = 40;    // walk targets
+constexpr int kMiscGroups = 41;     // walk units
+constexpr int kMiscQueue = 48;      // [48, 56): walk-unit queue heads, one per XCD
+constexpr int kMiscWords = 64;
+constexpr uint32_t kAbortNodes = 1;  // more nodes or leaves than the node arrays hold
+constexpr uint32_t kAbortDepth = 2;  // deeper than the levels the upward pass launched
 
 // The device a context was created on.
 int ctx_device(const bh_ctx* ctx);
@@ -81,6 +100,8 @@ struct TreeDev {
   uint32_t* leaf_node;   // [leaves] node of each leaf, Morton order
   uint32_t* leaf_first;  // [leaves] first particle of each leaf
   uint32_t* level_off;   // [kLevels + 2] level start offsets, then total, total
+  uint32_t* misc;        // [kMiscWords] the step's flags and counts
+  uint32_t cap;          // node (and leaf) capacity of the arrays above
 };
 
 struct WalkArgs {
@@ -97,12 +118,11 @@ struct WalkArgs {
   const SysDev* sys;       // system table; the root of system s is node s
   const double* nbox;      // [6 n_nodes] FP64 particle box of every node (lo xyz, hi xyz)
   double* gacc;            // [3n] scratch: a group's shared-list sums per particle
-  uint32_t* defer;         // [n_groups * kDefer] scratch: deferred nodes per group
-  uint32_t n_groups;
-  uint32_t max_depth;      // deepest level of the tree (sizes the walk's stack)
-  uint32_t* deep_frames;   // [n_groups * (kLevels - kShallowFrames) * 64] stack frames below the LDS ones
-  uint32_t n_nodes;        // index clamps: a node or particle index never
-  uint32_t n_parts;        // leaves its array, whatever the tree holds
+  uint32_t* defer;         // [slots * kDefer] scratch: deferred nodes, per walk workgroup
+  uint32_t* deep_frames;   // [slots * (kLevels - kShallowFrames) * 64] stack frames below the LDS ones
+  uint32_t* misc;          // the step's flags; unit count and queue heads
+  uint32_t n_nodes;        // index clamps (the node capacity): a node or particle
+  uint32_t n_parts;        // index never leaves its array, whatever the tree holds
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
   double* acc;             // [3n] G * acceleration (nullable)
   uint32_t* n_pp;          // [n] (nullable)
@@ -135,27 +155,36 @@ hipError_t launch_tree_count(const uint64_t* keys, uint32_t n, const uint16_t* s
                              uint32_t max_leaf_max, uint8_t* L, uint8_t* H, uint32_t* tile_counts, hipStream_t s);
 hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_t* tile_prefix, uint32_t* totals,
                             hipStream_t s);
-hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix,
-                            const uint32_t* totals, TreeDev t, hipStream_t s);
-// level_nodes: host array, nodes per level (sizes the upward launches)
-hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t max_depth,
-                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, const uint32_t* level_nodes, TreeDev t,
-                              hipStream_t s);
+// Numbers the nodes and writes their topology; when the tree has more nodes
+// or leaves than t.cap it writes nothing and sets kAbortNodes, when it is
+// deeper than depth_plan it sets kAbortDepth.
+hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix, TreeDev t,
+                            uint32_t depth_plan, hipStream_t s);
+// Links, geometry and the upward pass of levels depth_plan..0 (one launch
+// per level, each grid-striding over the level's range read on the device;
+// n_sys systems bound a level's node count by n_sys * 8^d).
+hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t n_sys,
+                              const double* lohi, uint32_t depth_plan, TreeDev t, hipStream_t s);
 // Marks (node + 1 at its first particle) and lists the topmost nodes with
 // leaf || count <= thr: thr = the system's tsm (walk targets) or, with
-// groups set, max(tsm, kGroupCap) (walk groups). With groups and unit_n,
-// runs of sibling single-target groups are packed into walk units (unit_n
-// per node: particles | 1 << 31 at a unit's first node, else 0).
-hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
-                          uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
-                          uint32_t* n_targets, uint32_t* unit_n, hipStream_t s);
+// groups set, max(tsm, tile) (walk groups; tile = the system's walk tile).
+// With groups and unit_n, runs of sibling single-target groups are packed
+// into walk units of at most tile particles (unit_n per node: particles |
+// 1 << 31 at a unit's first node, else 0).
+hipError_t launch_targets(TreeDev t, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups, uint32_t* mark,
+                          uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets,
+                          uint32_t* unit_n, hipStream_t s);
 
 // ---- bh_walk.hip
-hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s);
+// slots workgroups (a multiple of 8, at most kWalkSlots) take the units from
+// per-XCD queues; deep selects the variant whose stack also uses deep_frames
+// (trees of more than kShallowFrames levels).
+hipError_t launch_walk(const WalkArgs& a, uint32_t slots, bool deep, int fp64, bool any_unsoftened, hipStream_t s);
 hipError_t launch_direct(const double4* src, uint32_t n, const uint32_t* tidx, uint32_t m, double eps2, double G,
                          double* partial, uint32_t splits, double* acc, hipStream_t s);
 uint32_t direct_splits(uint32_t n, uint32_t m);
-// partial[splits * ceil(n/256)]: per (slice, block) sums of m_i * sum_j m_j / r_ij
+// partial[splits * ceil(n/256)]: per (slice, block) sums of m_i * m_j / r_ij over
+// unordered pairs {i, j} (each counted once, from the i of the block)
 hipError_t launch_potential(const double4* src, uint32_t n, double* partial, uint32_t splits, hipStream_t s);
 
 }  // namespace bhk

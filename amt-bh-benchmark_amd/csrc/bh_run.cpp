@@ -9,9 +9,12 @@
 // resident on the GPU between steps; they are copied to the host only for
 // energy reports and snapshots. Extra flags: --device, --fp64 (FP64 walk),
 // --output-csv PATH (final state, in Morton order like the reference's).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
+#include <iterator>
 #include <iostream>
 #include <random>
 #include <stdexcept>
@@ -39,37 +42,49 @@ struct Options {
   bool fp64 = false;
 };
 
+// The flags of main.cc:18-92 (plus this driver's own), each with the
+// setter of its option. A flag without a setter value is a switch.
 Options parse_args(int argc, char** argv) {
   Options o;
-  for (int i = 1; i < argc; ++i) {
-    const std::string a = argv[i];
-    auto next = [&]() -> std::string {
-      if (i + 1 >= argc) throw std::runtime_error("missing value for " + a);
-      return argv[++i];
-    };
-    if (a == "--input") o.input = next();
-    else if (a == "--steps") o.steps = std::stoi(next());
-    else if (a == "--dt") o.dt = std::stod(next());
-    else if (a == "--theta") o.theta = std::stod(next());
-    else if (a == "--eps") o.eps = std::stod(next());
-    else if (a == "--G") o.G = std::stod(next());
-    else if (a == "--max-leaf") o.max_leaf = static_cast<unsigned>(std::stoul(next()));
-    else if (a == "--target-size") o.target_size = static_cast<uint32_t>(std::stoul(next()));
-    else if (a == "--group-size") o.group_size = static_cast<uint32_t>(std::stoul(next()));
-    else if (a == "--no-quadrupole") o.use_quadrupole = false;
-    else if (a == "--vtk-prefix") {
-      o.vtk_prefix = next();
-      o.write_vtk = true;
-    } else if (a == "--energy-every") o.energy_every = std::stoi(next());
-    else if (a == "--time") o.time_steps = true;
-    else if (a == "--n") o.n = std::stoul(next());
-    else if (a == "--seed") o.seed = static_cast<unsigned>(std::stoul(next()));
-    else if (a == "--spread") o.spread = std::stod(next());
-    else if (a == "--vel-spread") o.vel_spread = std::stod(next());
-    else if (a == "--device") o.device = std::stoi(next());
-    else if (a == "--fp64") o.fp64 = true;
-    else if (a == "--output-csv") o.output_csv = next();
-    else throw std::runtime_error("unknown argument: " + a);
+  using Set = std::function<void(const std::string&)>;
+  struct Flag {
+    const char* name;
+    bool takes_value;
+    Set set;
+  };
+  const Flag flags[] = {
+      {"--input", true, [&](const std::string& v) { o.input = v; }},
+      {"--steps", true, [&](const std::string& v) { o.steps = std::stoi(v); }},
+      {"--dt", true, [&](const std::string& v) { o.dt = std::stod(v); }},
+      {"--theta", true, [&](const std::string& v) { o.theta = std::stod(v); }},
+      {"--eps", true, [&](const std::string& v) { o.eps = std::stod(v); }},
+      {"--G", true, [&](const std::string& v) { o.G = std::stod(v); }},
+      {"--max-leaf", true, [&](const std::string& v) { o.max_leaf = static_cast<unsigned>(std::stoul(v)); }},
+      {"--target-size", true, [&](const std::string& v) { o.target_size = static_cast<uint32_t>(std::stoul(v)); }},
+      {"--group-size", true, [&](const std::string& v) { o.group_size = static_cast<uint32_t>(std::stoul(v)); }},
+      {"--no-quadrupole", false, [&](const std::string&) { o.use_quadrupole = false; }},
+      {"--vtk-prefix", true, [&](const std::string& v) { o.vtk_prefix = v, o.write_vtk = true; }},
+      {"--energy-every", true, [&](const std::string& v) { o.energy_every = std::stoi(v); }},
+      {"--time", false, [&](const std::string&) { o.time_steps = true; }},
+      {"--n", true, [&](const std::string& v) { o.n = std::stoul(v); }},
+      {"--seed", true, [&](const std::string& v) { o.seed = static_cast<unsigned>(std::stoul(v)); }},
+      {"--spread", true, [&](const std::string& v) { o.spread = std::stod(v); }},
+      {"--vel-spread", true, [&](const std::string& v) { o.vel_spread = std::stod(v); }},
+      {"--device", true, [&](const std::string& v) { o.device = std::stoi(v); }},
+      {"--fp64", false, [&](const std::string&) { o.fp64 = true; }},
+      {"--output-csv", true, [&](const std::string& v) { o.output_csv = v; }},
+  };
+  int k = 1;
+  while (k < argc) {
+    const std::string flag = argv[k++];
+    const Flag* f = std::find_if(std::begin(flags), std::end(flags), [&](const Flag& x) { return flag == x.name; });
+    if (f == std::end(flags)) throw std::runtime_error("unknown argument: " + flag);
+    if (!f->takes_value) {
+      f->set(std::string());
+      continue;
+    }
+    if (k >= argc) throw std::runtime_error("missing value for " + flag);
+    f->set(argv[k++]);
   }
   return o;
 }

@@ -23,6 +23,14 @@
 //
 // Each system's arithmetic is independent of the others in the batch, so a
 // request's result is bit-identical to stepping it alone.
+//
+// Failures. A request with a non-finite value is refused at submission. If
+// the batched step fails anyway, every active system is stepped alone from
+// the same input (which the step never writes): the ones that succeed have
+// exactly the result the batch would have given them and carry on, the ones
+// that fail complete with the error as their status. If the buffers cannot
+// grow for newly admitted requests, those go back to the head of the queue.
+// Either way no request is lost and no footprint leaks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -138,6 +146,7 @@ bh_system system_of(const Active& a) {
   y.max_leaf = a.q.r.max_leaf;
   y.group_size = a.q.r.group_size;
   y.target_size = a.q.r.target_size;
+  y.walk_tile = 0;  // the context's
   kick_drift(a, &y.kick_dt, &y.drift_dt);
   return y;
 }
@@ -217,6 +226,12 @@ int bh_server_submit(bh_server* h, const bh_particle* particles, bh_particle* re
   if (r->max_leaf > bhk::kMaxLeafCap || r->group_size == 0)
     return fail(s, BH_ERR_INVALID, "max_leaf must be <= 1024 and group_size >= 1");
   if (r->n > bhk::kMaxN) return fail(s, BH_ERR_TOO_LARGE, "request exceeds the per-batch particle limit");
+  for (uint64_t i = 0; i < r->n; ++i) {
+    const bh_particle& q = particles[i];
+    if (!(std::isfinite(q.mass) && std::isfinite(q.pos[0]) && std::isfinite(q.pos[1]) && std::isfinite(q.pos[2]) &&
+          std::isfinite(q.vel[0]) && std::isfinite(q.vel[1]) && std::isfinite(q.vel[2])))
+      return fail(s, BH_ERR_NONFINITE, "particle " + std::to_string(i) + " has a non-finite mass, position or velocity");
+  }
   const uint64_t fp = bh_server_footprint(r);
   if (fp > s->cap) return fail(s, BH_ERR_TOO_LARGE, "request footprint exceeds the server's memory cap");
   Req q;
@@ -261,7 +276,15 @@ int bh_server_step(bh_server* h, bh_server_stats* stats) {
   const bool idle = s->active.empty();
   if (n_new) {
     const int rc = grow(s, n_act + n_new, n_act);
-    if (rc != BH_OK) return rc;
+    if (rc != BH_OK) {
+      // back to the head of the queue, in their order, with their footprint released
+      for (auto it = admitted.rbegin(); it != admitted.rend(); ++it) {
+        s->used -= it->q.footprint;
+        it->q.admit_s = 0.0;
+        s->pending.push_front(it->q);
+      }
+      return rc;
+    }
     // idle server: stage straight into the input and step now; otherwise
     // behind the active range of the output, overlapping this step
     bh_particle* dst = s->buf[idle ? s->in : 1 - s->in];
@@ -291,8 +314,18 @@ int bh_server_step(bh_server* h, bh_server_stats* stats) {
   bh_particle* out = s->buf[1 - s->in];
   bh_step_timing t{};
   const double t_step = now_s(s);
-  int rc = bh_step_batch(s->ctx, in, out, table.data(), static_cast<uint32_t>(table.size()), &t);
-  if (rc != BH_OK) return fail(s, rc, std::string("batched step: ") + bh_last_error(s->ctx));
+  std::vector<int> failed(s->active.size(), BH_OK);
+  const int rc = bh_step_batch(s->ctx, in, out, table.data(), static_cast<uint32_t>(table.size()), &t);
+  if (rc != BH_OK) {
+    // each system alone, from the same (unwritten) input
+    s->err = std::string("batched step: ") + bh_last_error(s->ctx);
+    for (size_t k = 0; k < s->active.size(); ++k) {
+      const Active& a = s->active[k];
+      const bh_system& y = table[k];
+      failed[k] = bh_step_device(s->ctx, in + a.off, out + a.off, y.n, &y.fp, y.max_leaf, y.group_size, y.target_size,
+                                 y.kick_dt, y.drift_dt, nullptr);
+    }
+  }
   SRV_HIP(s, hipEventSynchronize(s->staged_ev));
   const double t_end = now_s(s);
   for (auto& a : admitted) {
@@ -307,17 +340,19 @@ int bh_server_step(bh_server* h, bh_server_stats* stats) {
   std::vector<Active> keep;
   keep.reserve(s->active.size() + admitted.size());
   uint32_t completed = 0;
-  for (auto& a : s->active) {
-    a.phase += 1;
-    if (a.phase > a.q.r.steps) {
-      SRV_HIP(s, hipMemcpy(a.q.result, out + a.off, a.q.r.n * sizeof(bh_particle), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < s->active.size(); ++k) {
+    Active& a = s->active[k];
+    if (failed[k] == BH_OK) a.phase += 1;
+    if (a.phase > a.q.r.steps || failed[k] != BH_OK) {
+      if (failed[k] == BH_OK)
+        SRV_HIP(s, hipMemcpy(a.q.result, out + a.off, a.q.r.n * sizeof(bh_particle), hipMemcpyDeviceToHost));
       bh_completion c{};
       c.ticket = a.q.ticket;
       c.submit_s = a.q.submit_s;
       c.admit_s = a.q.admit_s;
       c.done_s = now_s(s);
       c.evaluations = a.phase;
-      c.status = BH_OK;
+      c.status = failed[k];
       s->done.push_back(c);
       s->used -= a.q.footprint;
       ++completed;

@@ -156,24 +156,40 @@ __global__ __launch_bounds__(1024) void k_col_scan(const uint32_t* __restrict__ 
   if (threadIdx.x == 0) totals[col] = carry;
 }
 
+// Node numbering and topology. Nothing is written when the tree has more
+// nodes or leaves than the arrays hold (kAbortNodes; the host grows them and
+// runs the step again); a tree deeper than the upward launches planned is
+// numbered but flagged (kAbortDepth), so the kernels after this one skip it.
 __global__ __launch_bounds__(kBlock) void k_tree_emit(const uint8_t* __restrict__ L, const uint8_t* __restrict__ H, uint32_t n,
-                                                      const uint32_t* __restrict__ tile_prefix,
-                                                      const uint32_t* __restrict__ totals, TreeDev t) {
+                                                      const uint32_t* __restrict__ tile_prefix, TreeDev t,
+                                                      uint32_t depth_plan) {
   __shared__ uint32_t s_loff[kLevels + 2];
   __shared__ uint32_t s_wc[4][kTreeCols];
+  __shared__ uint32_t s_over;
   const uint32_t tile = blockIdx.x, base = tile * kTreeTile;
   const int w = threadIdx.x >> 6, lane = __lane_id();
   if (threadIdx.x == 0) {
-    uint32_t off = 0;
+    const uint32_t* totals = t.misc + kMiscLevels;
+    uint32_t off = 0, deepest = 0;
     for (int d = 0; d < kLevels; ++d) {
       s_loff[d] = off;
       off += totals[d];
+      if (totals[d]) deepest = static_cast<uint32_t>(d);
     }
     s_loff[kLevels] = off;
     s_loff[kLevels + 1] = off;
-    if (tile == 0)
-      for (int d = 0; d < kLevels + 2; ++d) t.level_off[d] = s_loff[d];
+    const bool over = off > t.cap || totals[kLevels] > t.cap;
+    s_over = over ? 1u : 0u;
+    if (tile == 0) {
+      t.misc[kMiscMaxDepth] = deepest;
+      const uint32_t abort = (over ? kAbortNodes : 0u) | (deepest > depth_plan ? kAbortDepth : 0u);
+      if (abort) t.misc[kMiscAbort] = abort;
+      if (!over)
+        for (int d = 0; d < kLevels + 2; ++d) t.level_off[d] = s_loff[d];
+    }
   }
+  __syncthreads();
+  if (s_over) return;
   // phase 1: per-wave counts of level begins and leaf begins
   uint32_t c[kLevels + 1];
 #pragma unroll
@@ -248,7 +264,9 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit(const uint8_t* __restrict_
 // Children, parents, leaf counts and cell geometry of every node.
 __global__ __launch_bounds__(kBlock) void k_tree_links(const uint64_t* __restrict__ keys, uint32_t n,
                                                        const uint16_t* __restrict__ sid, const double* __restrict__ lohi,
-                                                       uint32_t n_nodes, uint32_t n_leaves, TreeDev t) {
+                                                       TreeDev t) {
+  if (t.misc[kMiscAbort]) return;
+  const uint32_t n_nodes = t.level_off[kLevels], n_leaves = t.misc[kMiscLevels + kLevels];
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
     const int d = t.depth[i];
     const uint32_t lend = t.level_off[d + 1];
@@ -287,6 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_tree_links(const uint64_t* __restric
 // finalize_node for every node of one level (flat_tree.h:257-274), children
 // already final. Also derives the walk's node records.
 __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ posm, int level, TreeDev t) {
+  if (t.misc[kMiscAbort]) return;
   const uint32_t lo = t.level_off[level], hi = t.level_off[level + 1];
   for (uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x; i < hi; i += gridDim.x * kBlock) {
     const double4 cc = t.ctr[i];
@@ -401,11 +420,16 @@ __global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ p
 // Marks each walk target at its first particle: topmost nodes with
 // leaf || count <= tsm (tsm = min(target_size, group_size), which makes the
 // group-root / target nesting of grouped_ttg.h:251-288 a single predicate).
-__global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_nodes, const uint16_t* __restrict__ sid,
-                                                        const SysDev* __restrict__ sys, uint32_t floor_thr,
+// With `groups`, the walk groups instead: topmost nodes with
+// leaf || count <= max(tsm, tile).
+__global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, const uint16_t* __restrict__ sid,
+                                                        const SysDev* __restrict__ sys, bool groups,
                                                         uint32_t* __restrict__ mark) {
+  if (t.misc[kMiscAbort]) return;
+  const uint32_t n_nodes = t.level_off[kLevels];
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
-    const uint32_t tsm = max(sys[sid[t.begin[i]]].tsm, floor_thr);
+    const SysDev& q = sys[sid[t.begin[i]]];
+    const uint32_t tsm = groups ? max(q.tsm, q.tile) : q.tsm;
     const bool pred = t.leaf[i] || t.count[i] <= tsm;
     if (!pred) continue;
     const bool ppred = (t.depth[i] != 0) && (t.count[t.parent[i]] <= tsm);
@@ -414,20 +438,24 @@ __global__ __launch_bounds__(kBlock) void k_target_mark(TreeDev t, uint32_t n_no
 }
 
 // Walk units from the walk groups. Below a node with more than
-// max(tsm, kGroupCap) particles, a run of consecutive children that are
-// single targets (leaf || count <= tsm) is packed greedily into units of at
-// most kGroupCap particles; a unit of two or more walks as one group (one
+// max(tsm, tile) particles (tile: the system's walk tile), a run of
+// consecutive children that are single targets (leaf || count <= tsm) is
+// packed greedily into units of at most tile particles; a unit of two or
+// more walks as one group (one
 // shared walk, then its targets), the others keep their own walks. The
 // unit's first child keeps its mark and gets unit_n = particles | 1 << 31,
 // the other members lose theirs; every other walk group gets unit_n = 0.
 // Any box holding the targets' boxes gives the same lists (near2 and far2
 // are monotone in the box), so a unit is exact whatever it groups.
-__global__ __launch_bounds__(kBlock) void k_group_units(TreeDev t, uint32_t n_nodes, const uint16_t* __restrict__ sid,
+__global__ __launch_bounds__(kBlock) void k_group_units(TreeDev t, const uint16_t* __restrict__ sid,
                                                         const SysDev* __restrict__ sys, uint32_t* __restrict__ mark,
                                                         uint32_t* __restrict__ unit_n) {
+  if (t.misc[kMiscAbort]) return;
+  const uint32_t n_nodes = t.level_off[kLevels];
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
-    const uint32_t tsm = sys[sid[t.begin[i]]].tsm;
-    const uint32_t thr = max(tsm, kGroupCap);
+    const SysDev& q = sys[sid[t.begin[i]]];
+    const uint32_t tsm = q.tsm, tile = q.tile;
+    const uint32_t thr = max(tsm, tile);
     const bool is_group = t.leaf[i] || t.count[i] <= thr;
     if (t.depth[i] == 0 && is_group) unit_n[i] = 0u;  // a system that is one walk group
     if (is_group) continue;
@@ -436,9 +464,9 @@ __global__ __launch_bounds__(kBlock) void k_group_units(TreeDev t, uint32_t n_no
     for (uint32_t c = fc; c <= fc + k; ++c) {
       const bool in = c < fc + k;
       const uint32_t cn = in ? t.count[c] : 0u;
-      const bool single = in && (t.leaf[c] || cn <= tsm) && cn <= kGroupCap;
+      const bool single = in && (t.leaf[c] || cn <= tsm) && cn <= tile;
       if (in && (t.leaf[c] || cn <= thr)) unit_n[c] = 0u;
-      if (single && run_len > 0 && run_n + cn <= kGroupCap) {
+      if (single && run_len > 0 && run_n + cn <= tile) {
         mark[t.begin[c]] = 0u;
         ++run_len;
         run_n += cn;
@@ -518,37 +546,39 @@ hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix, const uint32_t* totals,
-                            TreeDev t, hipStream_t s) {
+hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix, TreeDev t,
+                            uint32_t depth_plan, hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
-  hipLaunchKernelGGL(k_tree_emit, dim3(tiles), dim3(kBlock), 0, s, L, H, n, tile_prefix, totals, t);
+  hipLaunchKernelGGL(k_tree_emit, dim3(tiles), dim3(kBlock), 0, s, L, H, n, tile_prefix, t, depth_plan);
   return hipGetLastError();
 }
 
-hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t max_depth,
-                              const double* lohi, uint32_t n_nodes, uint32_t n_leaves, const uint32_t* level_nodes, TreeDev t,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_tree_links, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, keys, n, sid, lohi, n_nodes, n_leaves,
-                     t);
-  // bottom-up, one launch per non-empty level, sized by the level's node
-  // count; each reads its level range on the device
-  for (int d = static_cast<int>(max_depth < kDepthCap ? max_depth : kDepthCap); d >= 0; --d) {
-    hipLaunchKernelGGL(k_upward, dim3(grid_for(level_nodes[d], 2048)), dim3(kBlock), 0, s, posm, d, t);
+hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t n_sys,
+                              const double* lohi, uint32_t depth_plan, TreeDev t, hipStream_t s) {
+  // grid-stride loops over the node count read on the device
+  hipLaunchKernelGGL(k_tree_links, dim3(grid_for(t.cap, 4096)), dim3(kBlock), 0, s, keys, n, sid, lohi, t);
+  // bottom-up, one launch per level up to the planned depth; each reads its
+  // level's range on the device and grid-strides over it. Level d holds at
+  // most min(n_sys * 8^d, n) nodes, which sizes its grid.
+  const int top = static_cast<int>(depth_plan < kDepthCap ? depth_plan : kDepthCap);
+  for (int d = top; d >= 0; --d) {
+    uint64_t bound = n_sys;
+    for (int e = 0; e < d && bound < n; ++e) bound *= 8;
+    if (bound > n) bound = n;
+    hipLaunchKernelGGL(k_upward, dim3(grid_for(bound, 2048)), dim3(kBlock), 0, s, posm, d, t);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_targets(TreeDev t, uint32_t n_nodes, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups,
-                          uint32_t* mark, uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets,
-                          uint32_t* n_targets, uint32_t* unit_n, hipStream_t s) {
+hipError_t launch_targets(TreeDev t, uint32_t n, const uint16_t* sid, const SysDev* sys, bool groups, uint32_t* mark,
+                          uint32_t* tile_counts, uint32_t* tile_prefix, uint32_t* targets, uint32_t* n_targets,
+                          uint32_t* unit_n, hipStream_t s) {
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
   hipError_t e = launch_fill_u32(mark, 0u, n, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys,
-                     groups ? kGroupCap : 0u, mark);
+  hipLaunchKernelGGL(k_target_mark, dim3(grid_for(t.cap, 4096)), dim3(kBlock), 0, s, t, sid, sys, groups, mark);
   if (groups && unit_n)
-    hipLaunchKernelGGL(k_group_units, dim3(grid_for(n_nodes, 4096)), dim3(kBlock), 0, s, t, n_nodes, sid, sys, mark,
-                       unit_n);
+    hipLaunchKernelGGL(k_group_units, dim3(grid_for(t.cap, 4096)), dim3(kBlock), 0, s, t, sid, sys, mark, unit_n);
   hipLaunchKernelGGL(k_compact_count, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_counts);
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, 1u, tile_prefix, n_targets);
   hipLaunchKernelGGL(k_compact_write, dim3(tiles), dim3(kBlock), 0, s, mark, n, tile_prefix, targets);

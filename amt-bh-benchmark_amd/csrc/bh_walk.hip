@@ -638,8 +638,16 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
     const bool cellf = nv && accept;
     const bool deff = nv && defer;
     const bool leaff = nv && !accept && !defer && isleaf;
-    // the launch sizes the stack for the tree's depth; the guard keeps it in bounds regardless
-    const bool openf = nv && !accept && !defer && !isleaf && depth + 1 < kStack;
+    // An interior node to open needs a child and a free stack frame. The
+    // launch sizes the stack for the tree's depth, so neither check fails on
+    // a well-formed tree; if one does, the node is not opened and the step
+    // reports an incomplete walk (the reference's missing > 0,
+    // integrate_ttg.h:104-106).
+    const bool openable = !isleaf && lcnt != 0u && depth + 1 < kStack;
+    const bool openf = nv && !accept && !defer && !isleaf && openable;
+    if (__ballot(nv && !accept && !defer && !isleaf && !openable)) {
+      if (lane == 0) atomicOr(&a.misc[kMiscIncomplete], 1u);
+    }
     if constexpr (kGroup) {
       const uint64_t dm = __ballot(deff);
       if (dm) {
@@ -916,7 +924,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
 // unit's particles and keeping the deferred nodes; then each target of the
 // unit walks only those.
 template <typename Real, bool kMask, typename Lds>
-__device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g) {
+__device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g) {
   const int lane = __lane_id();
   const uint32_t gb = uniform(a.begin[g]);
   // a unit packed from sibling single-target groups spans several nodes
@@ -926,7 +934,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   const uint32_t sys_id = uniform(a.sid[gb]);
   const SysDev& sp = a.sys[sys_id];
   const bool gleaf = (a.rec[g].count >> 31) != 0;
-  uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(unit) * (kLevels - Lds::kFrames) * kWave : nullptr;
+  uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(slot) * (kLevels - Lds::kFrames) * kWave : nullptr;
   if (!packed && (gleaf || G <= sp.tsm || G > kGroupCap)) {
     unsigned long long tot_pp = 0, tot_pc = 0;
     walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0, tot_pp, tot_pc);
@@ -949,7 +957,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
     ev.gy[k] = static_cast<Real>(q.y - gcv.y);
     ev.gz[k] = static_cast<Real>(q.z - gcv.z);
   }
-  uint32_t* dl = a.defer + uint64_t(unit) * kDefer;
+  uint32_t* dl = a.defer + uint64_t(slot) * kDefer;
   uint32_t g_pp = 0, g_pc = 0, n_def = 0;
   const bool ok = traverse<Real, kMask, true>(a, L, gfr, box, gcv.x, gcv.y, gcv.z, sys_id, nullptr, 0,
                                               sp.use_quad != 0, sp.theta2, ev, g_pp, g_pc, dl, n_def);
@@ -1004,21 +1012,34 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t unit, uint32_t g)
   add_totals(a, tot_pp, tot_pc);
 }
 
-// Block b runs on XCD b % 8; give each XCD a contiguous run of groups so
-// neighbouring groups share that XCD's L2.
-__device__ inline uint32_t xcd_block(uint32_t b, uint32_t grid) {
-  const uint32_t per = grid / 8;
-  return (b % 8) * per + b / 8;
-}
-
+// A fixed grid of workgroups (one wavefront each) takes the walk units from
+// eight queues. Workgroup b runs on XCD b % 8 and takes units from queue
+// b % 8, which holds the eighth [x U / 8, (x + 1) U / 8) of the U units
+// (Morton order): each XCD walks a contiguous run, so neighbouring units
+// share its L2, and within an XCD the units go to whichever wavefront is
+// free. A wavefront leaves once its queue is past its end, so every
+// wavefront exits. The unit count and the queue heads are on the device
+// (counted and cleared by the step), so the launch needs no host read.
 template <typename Real, int kWaves, bool kMask, bool kDeep>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(WalkArgs a) {
   __shared__ WaveLds<Real, kShallowFrames, kDeep> lds[kWaves];
-  const int w = threadIdx.x >> 6;
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t t = blk * kWaves + w;
-  if (t >= a.n_groups) return;
-  walk_group<Real, kMask>(a, lds[w], t, a.groups[t]);
+  static_assert(kWaves == 1, "one wavefront per workgroup: the queues hand out units per workgroup");
+  if (a.misc[kMiscNonfinite] | a.misc[kMiscAbort]) return;
+  const uint32_t units = a.misc[kMiscGroups];
+  const uint32_t x = blockIdx.x & 7u;
+  const uint32_t lo = static_cast<uint32_t>(uint64_t(units) * x / 8);
+  const uint32_t hi = static_cast<uint32_t>(uint64_t(units) * (x + 1) / 8);
+  for (;;) {
+    // Every lane takes part in the claim (adding 1 for lane 0, 0 for the
+    // others) and the unit is read from lane 0's result: no branch on the
+    // lane id inside the loop, so all 64 lanes stay together in it (a
+    // lane-0-only claim lets the compiler keep the other lanes looping on
+    // their own, with a stale claim).
+    const uint32_t mine = atomicAdd(&a.misc[kMiscQueue + x], __lane_id() == 0 ? 1u : 0u);
+    const uint32_t k = uniform(__builtin_amdgcn_readlane(mine, 0));
+    if (k >= hi - lo) break;
+    walk_group<Real, kMask>(a, lds[0], blockIdx.x, uniform(a.groups[lo + k]));
+  }
 }
 
 // brute_force_accel (brute_force.h:15-28) for targets tidx[0..m) over a
@@ -1077,31 +1098,49 @@ __global__ __launch_bounds__(256) void k_direct_reduce(const double* __restrict_
   acc[3 * uint64_t(k) + 2] = G * a2;
 }
 
-// potential_energy (integrate.h:35-43): each target i sums m_j / r_ij over a
-// slice of the sources j != i; the per-(block, slice) partials of
-// m_i * sum are reduced in a fixed order, and E_P = -G/2 * total.
+// potential_energy (integrate.h:35-43) over unordered pairs: particle i
+// adds m_j / r_ij for the partners j = i + d (mod n), d = 1 .. floor(n / 2),
+// which counts every pair once (in an even-sized set the pair n / 2 apart is
+// counted from its lower index only) and gives every thread the same work.
+// Split s of the grid's second dimension takes the offsets
+// [1 + h s / splits, 1 + h (s + 1) / splits), h = floor(n / 2); a block
+// walks the window of partners its 256 particles share in tiles of 256.
+// partial[s * blocks + b] = sum over the block's i of m_i sum_j m_j / r_ij.
 __global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ src, uint32_t n,
                                                    double* __restrict__ partial, uint32_t splits) {
   __shared__ double4 s_src[256];
   __shared__ double s_red[4];
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t i0 = blockIdx.x * 256;
+  const uint32_t i = i0 + threadIdx.x;
   const uint32_t split = blockIdx.y;
-  const uint32_t lo = static_cast<uint32_t>(uint64_t(n) * split / splits);
-  const uint32_t hi = static_cast<uint32_t>(uint64_t(n) * (split + 1) / splits);
+  const uint64_t h = n / 2;
+  const uint64_t d_lo = 1 + h * split / splits, d_hi = 1 + h * (split + 1) / splits;
   const bool ok = i < n;
+  // this thread's partner offsets [d_lo, my_hi)
+  uint64_t my_hi = d_hi;
+  if (n % 2 == 0 && d_hi == h + 1 && i >= h) my_hi = h;
   double4 t = make_double4(0, 0, 0, 0);
   if (ok) t = src[i];
   double e = 0.0;
-  for (uint32_t b = lo; b < hi; b += 256) {
+  // partners of the block: positions i0 + d_lo .. i0 + 255 + d_hi - 1 (unwrapped)
+  const uint64_t w_lo = uint64_t(i0) + d_lo, w_hi = uint64_t(i0) + 255 + d_hi;
+  for (uint64_t jb = w_lo; jb < w_hi; jb += 256) {
     __syncthreads();
-    if (b + threadIdx.x < hi) s_src[threadIdx.x] = src[b + threadIdx.x];
+    {
+      const uint64_t j = jb + threadIdx.x;
+      if (j < w_hi) s_src[threadIdx.x] = src[j % n];
+    }
     __syncthreads();
-    const uint32_t cnt = hi - b < 256 ? hi - b : 256;
-    for (uint32_t jj = 0; jj < cnt; ++jj) {
-      if (b + jj == i) continue;
-      const double4 q = s_src[jj];
+    if (!ok) continue;
+    // unwrapped partner position of offset d is i + d
+    const int64_t k_lo = static_cast<int64_t>(uint64_t(i) + d_lo) - static_cast<int64_t>(jb);
+    const int64_t k_hi = static_cast<int64_t>(uint64_t(i) + my_hi) - static_cast<int64_t>(jb);
+    const int lo = static_cast<int>(k_lo < 0 ? 0 : (k_lo > 256 ? 256 : k_lo));
+    const int hi = static_cast<int>(k_hi < 0 ? 0 : (k_hi > 256 ? 256 : k_hi));
+    for (int k = lo; k < hi; ++k) {
+      const double4 q = s_src[k];
       const double dx = t.x - q.x, dy = t.y - q.y, dz = t.z - q.z;
-      e += q.w / sqrt(dx * dx + dy * dy + dz * dz);
+      e += q.w * rsqrt(dx * dx + dy * dy + dz * dz);
     }
   }
   double v = ok ? t.w * e : 0.0;
@@ -1114,13 +1153,12 @@ __global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ s
 // Masking r = 0 is needed only without softening (grouped_ttg.h:119-123); it
 // never triggers when eps2 > 0, so one masked launch serves mixed batches.
 template <typename Real, int kWaves>
-hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) {
-  uint32_t blocks = (a.n_groups + kWaves - 1) / kWaves;
-  blocks = (blocks + 7) / 8 * 8;  // a multiple of 8 keeps xcd_block a bijection
+hipError_t launch_walk_t(const WalkArgs& a, uint32_t slots, bool deep, bool any_unsoftened, hipStream_t s) {
+  // every XCD gets the same number of workgroups
+  const uint32_t blocks = (slots + 7) / 8 * 8;
   // The stack needs one frame per level below a walk's start. The first
   // kShallowFrames frames are in LDS (4 waves per SIMD); trees with more
   // levels (any system of a batch) also use the global deep frames.
-  const bool deep = a.max_depth + 1 > kShallowFrames;
   if (!any_unsoftened) {
     if (!deep)
       hipLaunchKernelGGL((k_walk<Real, kWaves, false, false>), dim3(blocks), dim3(kWaves * 64), 0, s, a);
@@ -1137,12 +1175,13 @@ hipError_t launch_walk_t(const WalkArgs& a, bool any_unsoftened, hipStream_t s) 
 
 }  // namespace
 
-hipError_t launch_walk(const WalkArgs& a, int fp64, bool any_unsoftened, hipStream_t s) {
-  if (a.n_groups == 0) return hipSuccess;
-  // One wavefront per workgroup: walk groups differ widely in work, and a
+hipError_t launch_walk(const WalkArgs& a, uint32_t slots, bool deep, int fp64, bool any_unsoftened, hipStream_t s) {
+  if (slots == 0 || slots % 8 != 0 || slots > kWalkSlots) return hipErrorInvalidValue;
+  // One wavefront per workgroup: walk units differ widely in work, and a
   // workgroup's LDS is released only when all its waves are done (2 waves
   // per workgroup: 49.7 ms at 16M, 4: 53.9, 8: 64.8; 1: 47.0).
-  return fp64 ? launch_walk_t<double, 1>(a, any_unsoftened, s) : launch_walk_t<float, 1>(a, any_unsoftened, s);
+  return fp64 ? launch_walk_t<double, 1>(a, slots, deep, any_unsoftened, s)
+              : launch_walk_t<float, 1>(a, slots, deep, any_unsoftened, s);
 }
 
 hipError_t launch_potential(const double4* src, uint32_t n, double* partial, uint32_t splits, hipStream_t s) {

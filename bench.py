@@ -14,8 +14,15 @@ Prints one JSON line. `roofline` is for the walk kernel (the dominant one):
 algorithmic FP32 flops per launch (19 per particle interaction, 55 per
 quadrupole-cell interaction) over its average duration from HIP events on
 the library's stream. `cpu_baseline` times the FP64 oracle (a C++
-restatement of the reference's grouped path) on the host's cores over a
-bounded sample of the same workload.
+restatement of the reference's grouped path) on the host's usable cores over
+a bounded sample of the same workload.
+
+Beside the headline the line carries each of the other workloads: the
+65,536-particle FP64 leapfrog (`config0`), the 1M uniform cube (`config1`),
+the 16M run's energy drift (`energy`), a 4,096-particle step's latency eager
+and as a graph replay (`small_system_latency`), the 64-system mixed
+ensemble batched and one system per call, each eager and graph-captured
+(`ensemble`), and the 2,000-request serving trace (`serving`).
 """
 import argparse
 import json
@@ -81,6 +88,49 @@ def profiled_traffic(args):
     return None
 
 
+def usable_cpus():
+    """CPUs this process may run on: its affinity set, capped by the cgroup
+    CPU quota when one is set. Returns (usable, affinity, quota or None)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, -(-int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return max(1, min(aff, quota) if quota else aff), aff, quota
+
+
+def host_info():
+    """CPU model, and the oracle's compiler and flags (oracle/Makefile)."""
+    import subprocess
+
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cxx = subprocess.run(["g++", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, IndexError, subprocess.SubprocessError):
+        cxx = "g++ (version not available on this host)"
+    flags = ""
+    try:
+        for line in open(os.path.join(REPO, "oracle", "Makefile")):
+            if line.startswith("CXXFLAGS"):
+                flags = line.split(":=", 1)[1].strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "compiler": cxx, "flags": flags}
+
+
 def cpu_baseline(p, fp, args, gpu_acc=None):
     """The FP64 oracle on the host's cores over a bounded sample; with the
     GPU's accelerations of the same tree (Morton order) also returns their
@@ -88,11 +138,7 @@ def cpu_baseline(p, fp, args, gpu_acc=None):
     sys.path.insert(0, REPO)
     import oracle  # the FP64 restatement, timed as the CPU baseline
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores, affinity, quota = usable_cpus()
     o = oracle.Oracle()
     (build_ms, walk_ms, frac), oacc, mask = o.baseline_sample(p, fp, args.max_leaf, args.group_size,
                                                               args.target_size, args.cpu_stride, cores, with_acc=True)
@@ -111,25 +157,33 @@ def cpu_baseline(p, fp, args, gpu_acc=None):
         "sample": (f"FP64 oracle, full tree build of all {len(p)} particles ({build_ms:.0f} ms) + grouped walk of "
                    f"every {args.cpu_stride}th group ({100 * frac:.2f}% of particles, {walk_ms:.0f} ms), walk time "
                    f"scaled to all groups"),
+        "threads": {"used": cores, "affinity": affinity, "cgroup_quota": quota},
+        "host": host_info(),
+        "note": "baseline only: the reference's own build needs TTG/PaRSEC, which are not available",
     }
 
 
 def ens_shapes(args, n, theta):
-    """(max_leaf, group_size, target_size): the autotuned table's choice for
-    (n, theta), unless fixed on the command line (--ens-max-leaf > 0)."""
+    """(max_leaf, group_size, target_size, walk_tile): the autotuned table's
+    choice for (n, theta), unless fixed on the command line (--ens-max-leaf > 0)."""
     if args.ens_max_leaf > 0:
-        return args.ens_max_leaf, args.ens_group_size, args.ens_target_size
+        return args.ens_max_leaf, args.ens_group_size, args.ens_target_size, 128
     from bhgpu import autotune
 
     return autotune.shapes_for(n, theta)
+
+
+def rel_rows(acc, ref):
+    return np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
 
 
 def ensemble_run(ctx, args):
     """The mixed-N ensemble (64 systems, N log-uniform in [4096, 2^20], half
     Plummer / half Hernquist, theta in {0.4, 0.5, 0.7}, eps 0.01): the
     batched path (bh_step_batch, every system in the same launches) against
-    the plain path (one bh_step_device call per system), each over the
-    leapfrog schedule of `ensemble_steps` steps from the same resident start."""
+    the plain path (one bh_step_device call per system), each eager and as
+    graph replays, each over the leapfrog schedule of `ensemble_steps` steps
+    from the same resident start."""
     from bhgpu import devmem, ensemble
 
     t0 = time.time()
@@ -145,6 +199,8 @@ def ensemble_run(ctx, args):
     buf = [devmem.DeviceBuffer(n * 64) for _ in range(2)]
 
     def run(mode, schedule):
+        batched = mode.startswith("batched")
+        ctx.set_graph_mode(bl.BH_GRAPH_ON if mode.endswith("graph") else bl.BH_GRAPH_OFF)
         buf[0].upload(p)
         devmem.synchronize()
         cur, lat = 0, []
@@ -152,33 +208,36 @@ def ensemble_run(ctx, args):
         for kick, drift in schedule:
             s0 = time.perf_counter()
             a, b = buf[cur].ptr, buf[1 - cur].ptr
-            if mode == "batched":
+            if batched:
                 ctx.step_batch_device(a, b, tables[(kick, drift)])
             else:
                 for k, m in enumerate(meta):
                     o = int(offs[k]) * 64
-                    ml, gs, ts = shapes(m)
+                    ml, gs, ts, tile = shapes(m)
+                    ctx.set_walk_tile(tile)
                     ctx.step_device(a + o, b + o, m["n"], fps[k], ml, gs, ts, kick, drift)
+                ctx.set_walk_tile(128)
             lat.append(time.perf_counter() - s0)
             cur ^= 1
         wall = time.perf_counter() - t_start
         return wall, lat, buf[cur].download(np.empty(n, layout.PARTICLE))
 
-    run("batched", sched[:1])  # grow the context's buffers
-    run("per_system", sched[:1])
-    res = {}
-    finals = {}
-    for mode in ("batched", "per_system"):
+    modes = ("batched", "batched_graph", "per_system", "per_system_graph")
+    for mode in modes:
+        run(mode, sched[:2])  # grow the context's buffers, capture both buffer directions
+    res, finals = {}, {}
+    for mode in modes:
         wall, lat, fin = run(mode, sched)
-        fin = fin.view(np.uint8)
-        finals[mode] = fin
+        finals[mode] = fin.view(np.uint8)
         res[mode] = {
             "particles_per_s": n * len(sched) / wall,
             "ms_per_step": 1e3 * wall / len(sched),
             "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         }
         log(f"ensemble {mode}: {res[mode]}")
-    identical = bool(np.array_equal(finals["batched"], finals["per_system"]))
+    graphs_held = ctx.graph_count()
+    ctx.set_graph_mode(bl.BH_GRAPH_OFF)
+    identical = all(np.array_equal(finals[m], finals["batched"]) for m in modes)
 
     # accuracy of the batched step: a = v_out - v_in from one step with
     # kick_dt = 1, drift_dt = 0, against the FP64 direct sum per system
@@ -188,18 +247,31 @@ def ensemble_run(ctx, args):
     ctx.step_batch_device(buf[0].ptr, buf[1].ptr, ensemble.table(meta, shapes, *kd))
     out = buf[1].download(np.empty(n, layout.PARTICLE))
     rng = np.random.default_rng(777)
-    errs, worst = [], 0.0
+    per_theta = {}
+    systems = []
     for k, m in enumerate(meta):
         lo, hi = int(offs[k]), int(offs[k + 1])
         ps, po = p[lo:hi], out[lo:hi]
         acc = np.empty((hi - lo, 3))
         acc[po["id"]] = po["vel"] - ps["vel"][po["id"]]
         idx = np.sort(rng.choice(hi - lo, size=min(args.ens_error_subset, hi - lo), replace=False)).astype(np.uint32)
-        ref = ctx.direct_sum(ps, idx, m["eps2"], 1.0)
-        rel = np.linalg.norm(acc[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-        errs.append(rel)
-        worst = max(worst, float(np.sqrt(np.mean(rel ** 2))))
-    rel = np.concatenate(errs)
+        rel = rel_rows(acc[idx], ctx.direct_sum(ps, idx, m["eps2"], 1.0))
+        rms = float(np.sqrt(np.mean(rel ** 2)))
+        systems.append({"n": m["n"], "kind": m["kind"], "theta": m["theta"], "shapes": list(shapes(m)), "rms_rel": rms,
+                        "p99_rel": float(np.percentile(rel, 99)), "within_bound": rms <= 1e-3})
+        per_theta.setdefault(m["theta"], []).append(rel)
+    accuracy = {"subset_per_system": args.ens_error_subset, "reference": "FP64 direct sum", "bound": 1e-3}
+    for th, rels in sorted(per_theta.items()):
+        r = np.concatenate(rels)
+        worst = max(s["rms_rel"] for s in systems if s["theta"] == th)
+        accuracy[f"theta_{th:g}"] = {"rms_rel": float(np.sqrt(np.mean(r ** 2))), "p99_rel": float(np.percentile(r, 99)),
+                                     "worst_system_rms_rel": worst,
+                                     "systems_within_bound": sum(s["within_bound"] for s in systems if s["theta"] == th),
+                                     "systems": sum(1 for s in systems if s["theta"] == th)}
+    allr = np.concatenate([np.concatenate(v) for v in per_theta.values()])
+    accuracy["all"] = {"rms_rel": float(np.sqrt(np.mean(allr ** 2))), "p99_rel": float(np.percentile(allr, 99)),
+                       "worst_system_rms_rel": max(s["rms_rel"] for s in systems)}
+    pps = lambda m: res[m]["particles_per_s"]  # noqa: E731
     return {
         "workload": f"ensemble_{len(meta)}x_loguniform_4096_1048576",
         "systems": len(meta),
@@ -209,14 +281,108 @@ def ensemble_run(ctx, args):
                    ", ".join(sorted({str(shapes(m)) for m in meta}))) if args.ens_max_leaf == 0 else
                   {"max_leaf": args.ens_max_leaf, "group_size": args.ens_group_size,
                    "target_size": args.ens_target_size},
-        "batched": res["batched"],
-        "per_system": res["per_system"],
-        "speedup": res["batched"]["particles_per_s"] / res["per_system"]["particles_per_s"],
-        "bit_identical_to_per_system": identical,
-        "accuracy": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "p99_rel": float(np.percentile(rel, 99)),
-                     "worst_system_rms_rel": worst, "subset_per_system": args.ens_error_subset,
-                     "reference": "FP64 direct sum"},
+        **res,
+        "speedup": pps("batched") / pps("per_system"),
+        "speedup_vs_graphed_per_system": max(pps("batched"), pps("batched_graph")) / pps("per_system_graph"),
+        "graphs_held": graphs_held,
+        "bit_identical_across_modes": identical,
+        "accuracy": accuracy,
+        "per_system_accuracy": systems,
     }
+
+
+def small_latency_run(ctx, n=4096, reps=50):
+    """One 4,096-particle Plummer step (leaf 16, group 256, target 32,
+    theta 0.5, eps 0.01) on device buffers: host wall time per call and the
+    device time of the step, eager and as a graph replay."""
+    from bhgpu import devmem
+
+    p = ic.plummer(n, 81)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    a, b = devmem.DeviceBuffer(n * 64), devmem.DeviceBuffer(n * 64)
+    a.upload(p)
+    out = {"workload": f"plummer_{n}", "n": n, "calls": reps}
+    for name, mode in (("eager", bl.BH_GRAPH_OFF), ("graph", bl.BH_GRAPH_ON)):
+        ctx.set_graph_mode(mode)
+        for _ in range(3):
+            ctx.step_device(a.ptr, b.ptr, n, fp, 16, 256, 32, 1e-3, 1e-3)
+        lat, dev = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.step_device(a.ptr, b.ptr, n, fp, 16, 256, 32, 1e-3, 1e-3)
+            lat.append(1e3 * (time.perf_counter() - t0))
+            dev.append(ctx.last_step_stats().step_ms)
+        out[name] = {"host_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                     "device_ms_p50": float(np.percentile(dev, 50)),
+                     "particles_per_s": n / (float(np.percentile(lat, 50)) * 1e-3)}
+    ctx.set_graph_mode(bl.BH_GRAPH_OFF)
+    return out
+
+
+def config0_run(ctx, args):
+    """Plummer N = 65,536 (seed 1), FP64 walk, eps 0.01, theta 0.5, leaf 16,
+    10 leapfrog steps of dt 1e-3 (11 force evaluations) on resident
+    particles; accelerations of the initial state against the FP64 direct
+    sum over all particles."""
+    n = 65536
+    p = ic.plummer(n, 1)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    ml, gs, ts, dt, steps = 16, 256, 32, 1e-3, 10
+    ctx.set_precision(bl.BH_FP64)
+    try:
+        ctx.build_tree(p, ml, gs)
+        acc, _, _, _ = ctx.accel(fp, ts)
+        perm = ctx.export_perm()
+        rel = rel_rows(acc, ctx.direct_sum(p[perm], None, 1e-4, 1.0))
+        sched = [(dt / 2, dt)] + [(dt, dt)] * (steps - 1) + [(dt / 2, 0.0)]
+        ctx.state_upload(p)
+        ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)  # warm-up (grows buffers), not part of the run
+        ctx.state_upload(p)
+        t0 = time.perf_counter()
+        for kick, drift in sched:
+            ctx.state_step(fp, ml, gs, ts, kick, drift)
+        wall = time.perf_counter() - t0
+    finally:
+        ctx.set_precision(bl.BH_FP64 if args.fp64 else bl.BH_FP32)
+    return {"workload": "plummer_65536_fp64_leapfrog10", "n": n, "evaluations": len(sched), "walk": "fp64",
+            "particles_per_s": n * len(sched) / wall, "ms_per_evaluation": 1e3 * wall / len(sched),
+            "accuracy": {"rms_rel": float(np.sqrt(np.mean(rel ** 2))), "p99_rel": float(np.percentile(rel, 99)),
+                         "subset": n, "reference": "FP64 direct sum over all particles (initial state)"}}
+
+
+def config1_run(ctx, args):
+    """Uniform cube N = 2^20 in [0,1)^3 (seed 2), eps 1e-3, theta 0.6, leaf 8
+    (the reference driver's default), one tree build + force evaluation;
+    FP32 and FP64 walk errors against the FP64 direct sum on a random
+    4,096-particle subset."""
+    n = 1 << 20
+    p = ic.uniform_cube(n, 2)
+    fp = layout.force_params(0.6, 1e-6, 1.0, True)
+    ml, gs, ts = 8, 256, 32
+    rng = np.random.default_rng(2)
+    idx = np.sort(rng.choice(n, size=4096, replace=False)).astype(np.uint32)
+    ref = ctx.direct_sum(p, idx, 1e-6, 1.0)
+    out = {"workload": "uniform_1048576_single_eval", "n": n, "theta": 0.6, "eps": 1e-3, "max_leaf": ml}
+    for name, prec in (("fp32", bl.BH_FP32), ("fp64", bl.BH_FP64)):
+        ctx.set_precision(prec)
+        ctx.build_tree(p, ml, gs)
+        acc, _, _, _ = ctx.accel(fp, ts)
+        where = np.empty(n, np.int64)
+        where[ctx.export_perm()] = np.arange(n)
+        rel = rel_rows(acc[where[idx]], ref)
+        ctx.state_upload(p)
+        ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)  # warm-up
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ctx.state_step(fp, ml, gs, ts, 0.0, 0.0)  # build + evaluation, no kick or drift
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        out[name] = {"particles_per_s": n / t, "ms_per_evaluation": 1e3 * t,
+                     "rms_rel": float(np.sqrt(np.mean(rel ** 2))), "p99_rel": float(np.percentile(rel, 99))}
+    ctx.set_precision(bl.BH_FP64 if args.fp64 else bl.BH_FP32)
+    out["accuracy_reference"] = "FP64 direct sum on 4,096 random particles"
+    return out
 
 
 def serving_run(ctx, args):
@@ -235,7 +401,7 @@ def serving_run(ctx, args):
     parts = [ic.plummer(int(n), 5000 + k) for k, n in enumerate(ns)]
     log(f"serving: {R} requests, {int(ns.sum())} particles, generated in {time.time() - t0:.1f}s")
     def req(n, k):
-        ml, gs, ts = ens_shapes(args, n, 0.5)
+        ml, gs, ts, _ = ens_shapes(args, n, 0.5)  # requests use the context's walk tile
         return bl.request(n, theta=0.5, eps2=1e-4, max_leaf=ml, group_size=gs, target_size=ts, steps=k, dt=1e-3)
 
     reqs = [req(int(n), int(k)) for n, k in zip(ns, steps)]
@@ -303,7 +469,7 @@ def main():
     ap.add_argument("--target-size", type=int, default=32)
     ap.add_argument("--fp64", action="store_true", help="FP64 walk arithmetic")
     ap.add_argument("--error-subset", type=int, default=4096)
-    ap.add_argument("--cpu-stride", type=int, default=128)
+    ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the mixed-N ensemble comparison")
     ap.add_argument("--ensemble-only", action="store_true")
@@ -311,11 +477,13 @@ def main():
     ap.add_argument("--ens-max-leaf", type=int, default=0, help="0: autotuned shapes per system")
     ap.add_argument("--ens-group-size", type=int, default=256)
     ap.add_argument("--ens-target-size", type=int, default=32)
-    ap.add_argument("--ens-error-subset", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=200, help="serving-trace prefix length (0: skip)")
+    ap.add_argument("--ens-error-subset", type=int, default=512)
+    ap.add_argument("--serve-requests", type=int, default=2000, help="serving-trace requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-cap-gb", type=float, default=200.0)
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--no-energy", action="store_true", help="skip the 16M run's O(N^2) energy drift")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 65k, 1M and 4,096-particle workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times, to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -361,6 +529,12 @@ def main():
 
     acc_report, gpu_acc = accuracy(ctx, p, fp, args)
     log(f"accuracy vs FP64 direct sum: {acc_report}")
+    energy = None
+    if not args.no_energy:
+        t0 = time.time()
+        ek0, ep0 = ctx.energies(p)
+        energy = {"kinetic_start": ek0, "potential_start": ep0, "potential_s": time.time() - t0}
+        log(f"energy at start: {energy}")
 
     ctx.state_upload(p)
     dt = args.dt
@@ -377,6 +551,15 @@ def main():
     wall = time.perf_counter() - t_start
     ctx.state_step(fp, args.max_leaf, args.group_size, args.target_size, dt / 2, 0.0)  # closing half-kick
     log(f"timed {args.steps} steps in {wall:.3f}s")
+    if energy is not None:
+        # main.cc:115-121: total energy after the run against the start; the
+        # closing half-kick brings velocities back in step with positions
+        ek1, ep1 = ctx.energies(ctx.state_download())
+        e0, e1 = energy["kinetic_start"] + energy["potential_start"], ek1 + ep1
+        energy.update({"kinetic_end": ek1, "potential_end": ep1, "total_start": e0, "total_end": e1,
+                       "drift_rel": (e1 - e0) / abs(e0), "leapfrog_steps": args.warmup + args.steps,
+                       "dt": dt, "potential": "unsoftened FP64 pair sum over all N(N-1)/2 pairs (integrate.h:35-43)"})
+        log(f"energy: {energy}")
 
     walk_ms = float(np.mean([s["walk_ms"] for s in stats]))
     flops = float(np.mean([FLOP_PP * s["n_pp"] + FLOP_PC * s["n_pc"] for s in stats]))
@@ -426,6 +609,15 @@ def main():
                               "factors from a 1 GiB device copy)",
         },
     }
+    if energy is not None:
+        out["energy"] = energy
+    if not args.no_configs:
+        out["config0"] = config0_run(ctx, args)
+        log(f"config0: {out['config0']}")
+        out["config1"] = config1_run(ctx, args)
+        log(f"config1: {out['config1']}")
+        out["small_system_latency"] = small_latency_run(ctx)
+        log(f"small system: {out['small_system_latency']}")
     if not args.no_ensemble:
         del p
         out["ensemble"] = ensemble_run(ctx, args)

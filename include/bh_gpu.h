@@ -69,7 +69,8 @@ enum bh_status {
   BH_OK = 0,
   BH_ERR_INVALID = 1,     /* bad argument (null pointer, group_size 0, ...) */
   BH_ERR_NONFINITE = 2,   /* a position, velocity or mass is NaN or infinite */
-  BH_ERR_INCOMPLETE = 3,  /* the walk opened a node without particles or children */
+  BH_ERR_INCOMPLETE = 3,  /* the walk met an interior node it could not open (no children, or deeper
+                             than the walk's stack); cannot occur on a tree this library built */
   BH_ERR_OOM = 4,         /* device allocation failed */
   BH_ERR_HIP = 5,         /* HIP runtime error */
   BH_ERR_TOO_LARGE = 6,   /* n or max_leaf beyond the supported range */
@@ -102,6 +103,31 @@ int bh_set_precision(bh_ctx* ctx, int precision);
  * every target. */
 typedef enum bh_walk_mode { BH_WALK_GROUPED = 0, BH_WALK_PER_TARGET = 1 } bh_walk_mode;
 int bh_set_walk_mode(bh_ctx* ctx, int mode);
+/* Graph mode (no reference counterpart; the reference's step is a TTG task
+ * graph rebuilt every call). A step makes no host decision between its first
+ * and last kernel, so with BH_GRAPH_ON every bh_step* call runs its launches
+ * as one HIP graph: captured the first time a given launch sequence occurs
+ * (same buffers, particle count, system count and table size, precision,
+ * walk mode and planned tree shape) and replayed after that; the system
+ * table (per-system parameters, kick and drift) is re-read on every replay.
+ * Results are bit-identical to BH_GRAPH_OFF (the default). The context keeps
+ * the 256 most recently used graphs; switching the mode off releases them.
+ * bh_graph_count reports how many are held. */
+typedef enum bh_graph_mode { BH_GRAPH_OFF = 0, BH_GRAPH_ON = 1 } bh_graph_mode;
+int bh_set_graph_mode(bh_ctx* ctx, int mode);
+/* Walk tile (no reference counterpart): the most particles one walk unit
+ * (one wavefront's shared walk) holds, BH_MIN_WALK_TILE..BH_MAX_WALK_TILE,
+ * default BH_MAX_WALK_TILE. Groups of the reference's group_size are split
+ * into units of at most this many particles, and runs of small sibling
+ * targets are packed into units up to it. Results do not depend on it (every
+ * target gets its own walk's interaction list either way; sums may round
+ * differently); smaller tiles give small systems more wavefronts, larger
+ * ones share more of the walk. Applies to the single-system entry points
+ * and to batch systems whose walk_tile is 0. */
+#define BH_MIN_WALK_TILE 16u
+#define BH_MAX_WALK_TILE 128u
+int bh_set_walk_tile(bh_ctx* ctx, uint32_t tile);
+int bh_graph_count(bh_ctx* ctx, uint32_t* count);
 /* Limits: n < 2^30 particles, max_leaf <= 1024. */
 uint64_t bh_max_particles(void);
 
@@ -116,8 +142,9 @@ int bh_step(bh_ctx* ctx, bh_particle* particles, uint64_t n, const bh_force_para
             uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt, bh_step_timing* timing);
 
 /* The same on device memory: reads d_in[0..n), writes the Morton-ordered,
- * updated particles to d_out[0..n). d_in and d_out must not overlap. The work
- * is ordered on the context's stream; the call returns when it is complete. */
+ * updated particles to d_out[0..n). d_in and d_out must not overlap; d_in is
+ * never written. The work is ordered on the context's stream and the host
+ * waits once, when it is complete. */
 int bh_step_device(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
                    uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
                    bh_step_timing* timing);
@@ -130,7 +157,7 @@ typedef struct bh_system {
   uint32_t max_leaf;
   uint32_t group_size;
   uint32_t target_size;
-  uint32_t pad;
+  uint32_t walk_tile; /* 0: the context's walk tile (bh_set_walk_tile) */
   double kick_dt;
   double drift_dt;
 } bh_system;
@@ -247,6 +274,10 @@ typedef struct bh_step_stats {
   uint64_t n_nodes;
   uint64_t n_leaves;
   uint64_t n_targets;
+  double step_ms;     /* the whole step on the device */
+  uint64_t n_units;   /* walk units (groups, packed runs of small groups, or targets) */
+  uint32_t graph;     /* 1: the step ran as a graph replay; the four phase times are then 0 */
+  uint32_t reruns;    /* times the step ran again because the tree outgrew the planned arrays or levels */
 } bh_step_stats;
 int bh_last_step_stats(bh_ctx* ctx, bh_step_stats* stats);
 

@@ -262,6 +262,12 @@ class Reference:
         lib.ref_kinetic_energy.restype = _D
         lib.ref_potential_energy.argtypes = [_P, _U64, _D]
         lib.ref_potential_energy.restype = _D
+        lib.ref_pn_build.argtypes = [_P, _U64, _U32, ctypes.POINTER(_P)]
+        lib.ref_pn_free.argtypes = [_P]
+        lib.ref_pn_num_leaves.argtypes = [_P]
+        lib.ref_pn_num_leaves.restype = _U64
+        lib.ref_pn_leaves.argtypes = [_P, _P, _P, _P, _P]
+        lib.ref_pn_pass.argtypes = [_P, _D, _D, _D, _I, _P, _P, _P]
         self.lib = lib
 
     def domain(self, p):
@@ -280,6 +286,15 @@ class Reference:
         if self.lib.ref_build(_ptr(p), len(p), max_leaf, group_size, ctypes.byref(h)) != 0:
             raise ValueError("empty input")
         return RefTree(self.lib, h, len(p))
+
+    def per_node_tree(self, p, max_leaf):
+        """The reference's per-node octree (tree.h/tree.cc) with its
+        multipoles; ids of p must be 0..n-1."""
+        p = np.ascontiguousarray(p)
+        h = _P()
+        if self.lib.ref_pn_build(_ptr(p), len(p), max_leaf, ctypes.byref(h)) != 0:
+            raise ValueError("empty input")
+        return RefPerNodeTree(self.lib, h, len(p))
 
     def brute_force(self, p, eps2, G=1.0):
         acc = np.zeros((len(p), 3))
@@ -306,3 +321,37 @@ class Reference:
 
 def reference_available():
     return os.path.exists(REF_SO)
+
+
+class RefPerNodeTree:
+    """bh::Tree of the reference's per-node path, built by its own tree.cc."""
+
+    def __init__(self, lib, h, n):
+        self._lib, self._h, self.n = lib, h, n
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.ref_pn_free(self._h)
+            self._h = None
+
+    def leaves(self):
+        """Leaves in octant (Morton) order: level, translation (k, 3),
+        count, and the concatenated ascending ids of each leaf."""
+        k = self._lib.ref_pn_num_leaves(self._h)
+        level = np.zeros(k, np.uint32)
+        txyz = np.zeros((k, 3), np.int32)
+        count = np.zeros(k, np.uint32)
+        ids = np.zeros(self.n, np.uint32)
+        self._lib.ref_pn_leaves(self._h, _ptr(level), _ptr(txyz), _ptr(count), _ptr(ids))
+        return level, txyz, count, ids
+
+    def per_node_pass(self, fp):
+        """Every leaf a target (traversal_ttg.h:156-196): (acc, n_pp, n_pc) by particle id."""
+        acc = np.zeros((self.n, 3))
+        npp = np.zeros(self.n, np.uint32)
+        npc = np.zeros(self.n, np.uint32)
+        rc = self._lib.ref_pn_pass(self._h, float(fp["theta"][0]), float(fp["eps2"][0]), float(fp["G"][0]),
+                                   int(fp["use_quadrupole"][0]), _ptr(acc), _ptr(npp), _ptr(npc))
+        if rc != 0:
+            raise ValueError("particle ids must be 0..n-1")
+        return acc, npp, npc

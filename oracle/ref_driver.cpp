@@ -14,20 +14,29 @@
 //   bh::accel_from_source                     (multipole.h:171-200)
 //   bh::brute_force_accel                     (brute_force.h:15-28)
 //   bh::kick_one / drift_one / energies       (integrate.h:15-43)
+//   bh::Tree (the per-node path's octree)     (tree.h:76-132, tree.cc:30-100)
+//   bh::Key / box_geometry                    (key.h, domain.h:59-67)
 // What this driver supplies itself (the reference's versions live in headers
 // that need the TTG runtime): the (key, index) sort, the serial loop that
 // orders backbone split / subtree count / subtree build / backbone finalize
-// as flat_build_ttg.h:194-273 does, and a per-target walk loop.
+// as flat_build_ttg.h:194-273 does, a per-target walk loop, and for the
+// per-node path the upward pass (upward_ttg.h:75-118: leaf raw moments about
+// the leaf's own box centre, interior sums of the children's shifted
+// moments, here in octant order where TTG sums them in arrival order) and
+// the dispatch recursion (traversal_ttg.h:156-196) as one serial walk per
+// target leaf.
 #include "brute_force.h"
 #include "flat_tree.h"
 #include "integrate.h"
 #include "multipole.h"
+#include "tree.h"
 
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <unordered_map>
 #include <vector>
 
 static_assert(sizeof(bh::Particle) == 64, "Particle layout");
@@ -44,6 +53,39 @@ struct RefTree {
 };
 
 std::vector<bh::Particle> to_vec(const bh::Particle* p, std::uint64_t n) { return std::vector<bh::Particle>(p, p + n); }
+
+// The per-node path's tree and its multipoles (interior nodes and leaves
+// other than a lone root leaf).
+struct PerNodeTree {
+  bh::Tree tree;
+  std::unordered_map<bh::Key, bh::Multipole> mp;
+  PerNodeTree(std::vector<bh::Particle> p, unsigned max_leaf) : tree(std::move(p), max_leaf) {}
+};
+
+// upward_ttg.h:75-118 for the subtree at `key`; returns its multipole.
+bh::Multipole upward(PerNodeTree& t, const bh::Key& key) {
+  const bh::Node& nd = t.tree.node(key);
+  const bh::Vec3 ref = t.tree.box_geometry(key).first;
+  bh::RawMoment sum;
+  if (nd.is_leaf()) {
+    const auto& ps = t.tree.particles();
+    for (std::uint32_t i = nd.particle_begin; i < nd.particle_begin + nd.particle_count; ++i)
+      sum += bh::raw_moment_of_particle(ps[i].mass, ps[i].pos, ref);
+  } else {
+    t.tree.for_each_child(key, [&](const bh::Key& ch) { sum += bh::raw_moment_from_multipole(upward(t, ch), ref); });
+  }
+  const bh::Multipole m = bh::finalize_raw_moment(sum, ref);
+  if (!(nd.is_leaf() && key == t.tree.root())) t.mp[key] = m;
+  return m;
+}
+
+void leaves_in_order(const bh::Tree& tree, const bh::Key& key, std::vector<bh::Key>& out) {
+  if (tree.is_leaf(key)) {
+    out.push_back(key);
+    return;
+  }
+  tree.for_each_child(key, [&](const bh::Key& ch) { leaves_in_order(tree, ch, out); });
+}
 
 }  // namespace
 
@@ -262,6 +304,104 @@ void ref_kick_drift(bh::Particle* p, std::uint64_t n, const double* acc, double 
     bh::kick_one(p[i], bh::Vec3{acc[3 * i], acc[3 * i + 1], acc[3 * i + 2]}, kick_dt);
     if (drift_dt != 0.0) bh::drift_one(p[i], drift_dt);
   }
+}
+
+// ---- the per-node path (tree.h / tree.cc, traversal_ttg.h:156-196)
+
+int ref_pn_build(const bh::Particle* p, std::uint64_t n, unsigned max_leaf, void** out) {
+  if (n == 0) return 1;
+  auto* t = new PerNodeTree(to_vec(p, n), max_leaf);
+  upward(*t, t->tree.root());
+  *out = t;
+  return 0;
+}
+
+void ref_pn_free(void* h) { delete static_cast<PerNodeTree*>(h); }
+
+std::uint64_t ref_pn_num_leaves(const void* h) {
+  const PerNodeTree* t = static_cast<const PerNodeTree*>(h);
+  std::uint64_t k = 0;
+  for (const auto& kv : t->tree.nodes()) k += kv.second.is_leaf() ? 1 : 0;
+  return k;
+}
+
+// Leaves in octant (Morton) order: level, translation x/y/z, particle count,
+// and the ids of each leaf's particles, ascending, concatenated.
+int ref_pn_leaves(const void* h, std::uint32_t* level, std::int32_t* txyz, std::uint32_t* count, std::uint32_t* ids) {
+  const PerNodeTree* t = static_cast<const PerNodeTree*>(h);
+  std::vector<bh::Key> lv;
+  leaves_in_order(t->tree, t->tree.root(), lv);
+  const auto& ps = t->tree.particles();
+  std::uint64_t pos = 0;
+  for (std::size_t k = 0; k < lv.size(); ++k) {
+    const bh::Node& nd = t->tree.node(lv[k]);
+    level[k] = lv[k].level();
+    for (int a = 0; a < 3; ++a) txyz[3 * k + a] = lv[k].translation()[a];
+    count[k] = nd.particle_count;
+    std::vector<std::uint32_t> v;
+    for (std::uint32_t i = nd.particle_begin; i < nd.particle_begin + nd.particle_count; ++i) v.push_back(ps[i].id);
+    std::sort(v.begin(), v.end());
+    std::memcpy(ids + pos, v.data(), v.size() * 4);
+    pos += v.size();
+  }
+  return 0;
+}
+
+// Every leaf a target: the dispatch recursion from the root with the
+// target's particle box, a source other than the target tested by the MAC
+// (accepted: its multipole), else a leaf source summed directly skipping
+// the particle itself, else its children. acc (G applied by
+// accel_from_source), n_pp (particles of the leaves summed, the target's
+// own included) and n_pc (accepted sources) are indexed by particle id
+// (ids must be 0..n-1).
+int ref_pn_pass(const void* h, double theta, double eps2, double G, int use_quadrupole, double* acc, std::uint32_t* n_pp,
+                std::uint32_t* n_pc) {
+  const PerNodeTree* t = static_cast<const PerNodeTree*>(h);
+  const bh::Tree& tree = t->tree;
+  const auto& ps = tree.particles();
+  const std::uint64_t n = ps.size();
+  for (const auto& q : ps)
+    if (q.id >= n) return 1;
+  std::vector<bh::Key> lv;
+  leaves_in_order(tree, tree.root(), lv);
+  for (const bh::Key& tk : lv) {
+    const bh::Node& tn = tree.node(tk);
+    bh::BBox box;
+    for (std::uint32_t i = tn.particle_begin; i < tn.particle_begin + tn.particle_count; ++i)
+      box.add(ps[i].pos.x, ps[i].pos.y, ps[i].pos.z);
+    for (std::uint32_t i = tn.particle_begin; i < tn.particle_begin + tn.particle_count; ++i) {
+      const bh::Particle& tp = ps[i];
+      std::uint32_t pp = 0, pc = 0;
+      auto visit = [&](auto&& self, const bh::Key& s) -> bh::Vec3 {
+        if (!(s == tk)) {
+          const bh::Multipole& m = t->mp.at(s);
+          if (bh::mac_accept_bbox(box, m.com, 2.0 * tree.box_geometry(s).second, theta)) {
+            ++pc;
+            return bh::accel_from_source(tp.pos, m.mass, m.com, m.iab, eps2, use_quadrupole != 0, G);
+          }
+        }
+        bh::Vec3 a{};
+        if (tree.is_leaf(s)) {
+          const bh::Node& sn = tree.node(s);
+          for (std::uint32_t j = sn.particle_begin; j < sn.particle_begin + sn.particle_count; ++j) {
+            ++pp;
+            if (ps[j].id == tp.id) continue;
+            a += bh::accel_from_source(tp.pos, ps[j].mass, ps[j].pos, bh::Sym3{}, eps2, false, G);
+          }
+          return a;
+        }
+        tree.for_each_child(s, [&](const bh::Key& ch) { a += self(self, ch); });
+        return a;
+      };
+      const bh::Vec3 a = visit(visit, tree.root());
+      acc[3 * tp.id + 0] = a.x;
+      acc[3 * tp.id + 1] = a.y;
+      acc[3 * tp.id + 2] = a.z;
+      n_pp[tp.id] = pp;
+      n_pc[tp.id] = pc;
+    }
+  }
+  return 0;
 }
 
 double ref_kinetic_energy(const bh::Particle* p, std::uint64_t n) { return bh::kinetic_energy(to_vec(p, n)); }

@@ -55,14 +55,16 @@ def test_struct_layouts_match_c(tmp_path):
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "bh_gpu.h"\n'
         "int main(void) {\n"
-        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bh_particle), sizeof(bh_force_params),\n'
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bh_particle), sizeof(bh_force_params),\n'
         "         sizeof(bh_system), offsetof(bh_system, kick_dt), sizeof(bh_request), offsetof(bh_request, dt),\n"
-        "         sizeof(bh_completion), sizeof(bh_server_stats), sizeof(bh_walk_node));\n"
+        "         sizeof(bh_completion), sizeof(bh_server_stats), sizeof(bh_walk_node), sizeof(bh_step_stats),\n"
+        "         offsetof(bh_step_stats, step_ms), offsetof(bh_step_stats, reruns));\n"
         "  return 0;\n}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c11", "-I", inc, "-o", str(exe), str(src)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [layout.PARTICLE.itemsize, layout.FORCE_PARAMS.itemsize, layout.SYSTEM.itemsize,
             layout.SYSTEM.fields["kick_dt"][1], ctypes.sizeof(bl.Request), bl.Request.dt.offset,
-            ctypes.sizeof(bl.Completion), ctypes.sizeof(bl.ServerStats), layout.WALK_NODE.itemsize]
+            ctypes.sizeof(bl.Completion), ctypes.sizeof(bl.ServerStats), layout.WALK_NODE.itemsize,
+            ctypes.sizeof(bl.StepStats), bl.StepStats.step_ms.offset, bl.StepStats.reruns.offset]
     assert got == want

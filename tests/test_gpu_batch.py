@@ -106,3 +106,22 @@ def test_batch_errors(gpu):
     out, _ = gpu.step_batch(p, good)
     alone, _ = gpu.step(p, layout.force_params(0.5, 1e-4), 8, 64, 32, 1e-3, 1e-3)
     assert out.tobytes() == alone.tobytes()
+
+
+def test_batch_walk_tiles_equal_alone(gpu):
+    """Systems of one batch with their own walk tiles each get the bytes of
+    a step alone with that tile."""
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    parts = [ic.plummer(6000, 201), ic.hernquist(9000, 202), ic.plummer(4096, 203)]
+    tiles = [16, 64, 0]
+    table = np.concatenate([layout.system(len(q), fp, 8, 256, 32, 1e-3, 1e-3, walk_tile=t) for q, t in zip(parts, tiles)])
+    out, _ = gpu.step_batch(np.concatenate(parts), table)
+    off = 0
+    try:
+        for q, t in zip(parts, tiles):
+            gpu.set_walk_tile(t or 128)
+            alone, _ = gpu.step(q, fp, 8, 256, 32, 1e-3, 1e-3)
+            assert out[off:off + len(q)].tobytes() == alone.tobytes(), t
+            off += len(q)
+    finally:
+        gpu.set_walk_tile(128)

@@ -57,8 +57,11 @@ def test_driver_random_ics_and_errors(tmp_path):
     assert r.returncode == 1 and "malformed line" in r.stderr
 
 
-def test_energies_match_oracle(gpu, orc):
-    p = ic.random_box(4000, 3, 5.0, vel_spread=0.3)
+@pytest.mark.parametrize("n", [2, 3, 257, 4000, 4001, 20001])
+def test_energies_match_oracle(gpu, orc, n):
+    # odd and even sizes: the device sum counts each unordered pair once
+    # (partners i + 1 .. i + n/2, the pair n/2 apart only from its lower index)
+    p = ic.random_box(n, 3, 5.0, vel_spread=0.3)
     ek, ep = gpu.energies(p)
     assert ek == pytest.approx(orc.kinetic_energy(p), rel=1e-14)
     assert ep == pytest.approx(orc.potential_energy(p), rel=1e-12)

@@ -336,3 +336,27 @@ def test_deterministic_replay(gpu):
     a, _ = gpu.step(p, fp, 16, 256, 32, 1e-3, 1e-3)
     b, _ = gpu.step(p, fp, 16, 256, 32, 1e-3, 1e-3)
     assert golden_io.bytes_equal(a, b)
+
+
+@pytest.mark.parametrize("tile", [16, 32, 48, 64, 96, 128])
+def test_walk_tile_keeps_lists(gpu, orc, tile):
+    """The walk tile (particles per walk unit) changes how targets share
+    walks, never what a target interacts with: lists exact, forces within
+    the precision's bar, for every tile."""
+    p = ic.plummer(30000, 91)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    oa, opp, opc, otot = orc.build(p, 8, 256).grouped_pass(fp, 32)
+    gpu.set_walk_tile(tile)
+    try:
+        gpu.build_tree(p, 8, 256)
+        for prec in (bl.BH_FP64, bl.BH_FP32):
+            print(f"walk tile {tile} prec={prec}", flush=True)
+            gpu.set_precision(prec)
+            ga, gpp, gpc, gtot = gpu.accel(fp, 32)
+            assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc) and np.array_equal(gtot, otot)
+            assert rel_err(ga, oa) < TOL[prec]
+    finally:
+        gpu.set_walk_tile(128)
+        gpu.set_precision(bl.BH_FP32)
+    with pytest.raises(bl.BHError):
+        gpu.set_walk_tile(8)

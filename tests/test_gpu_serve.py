@@ -100,3 +100,25 @@ def test_server_rejects_bad_requests(gpu):
     st = srv.step()  # nothing queued: a no-op
     assert st.active_systems == 0 and st.completed == 0
     srv.close()
+
+
+def test_server_refuses_nonfinite_request(gpu):
+    """A request holding a NaN is refused at submission (BH_ERR_NONFINITE);
+    the requests around it complete bit-identical to their runs alone."""
+    srv = bl.Server(gpu)
+    good = [(ic.plummer(2000, 21), bl.request(2000, 0.5, 1e-4, steps=2, max_leaf=16)),
+            (ic.hernquist(5000, 22), bl.request(5000, 0.6, 1e-4, steps=3, max_leaf=8))]
+    bad = ic.plummer(3000, 23)
+    bad["vel"][17, 1] = np.nan
+    tickets = {srv.submit(*good[0]): 0}
+    with pytest.raises(bl.BHError) as e:
+        srv.submit(bad, bl.request(3000, 0.5, 1e-4, steps=2))
+    assert e.value.code == 2
+    srv.step()
+    tickets[srv.submit(*good[1])] = 1
+    got = drain(srv)
+    assert sorted(got) == sorted(tickets)
+    for t, k in tickets.items():
+        p, req = good[k]
+        assert got[t][0].tobytes() == alone(gpu, p, req).tobytes(), k
+    srv.close()

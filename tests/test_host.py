@@ -34,12 +34,13 @@ def test_ensemble_small_make_and_table():
 def test_autotune_lookup(tmp_path):
     path = tmp_path / "t.json"
     assert autotune.shapes_for(1000, 0.5, str(path)) == autotune.DEFAULT
-    json.dump({"classes": [{"n": 4096, "theta": 0.5, "shapes": [8, 64, 16]},
-                           {"n": 4096, "theta": 0.7, "shapes": [16, 64, 16]},
+    json.dump({"classes": [{"n": 4096, "theta": 0.5, "shapes": [8, 64, 16, 32]},
+                           {"n": 4096, "theta": 0.7, "shapes": [16, 64, 16, 64]},
                            {"n": 1 << 20, "theta": 0.5, "shapes": [32, 256, 32]}]}, open(path, "w"))
-    assert autotune.shapes_for(5000, 0.5, str(path)) == (8, 64, 16)
-    assert autotune.shapes_for(5000, 0.7, str(path)) == (16, 64, 16)
-    assert autotune.shapes_for(500000, 0.4, str(path)) == (32, 256, 32)
+    assert autotune.shapes_for(5000, 0.5, str(path)) == (8, 64, 16, 32)
+    assert autotune.shapes_for(5000, 0.7, str(path)) == (16, 64, 16, 64)
+    # a table without the tile axis gives the largest tile
+    assert autotune.shapes_for(500000, 0.4, str(path)) == (32, 256, 32, autotune.MAX_TILE)
 
 
 def test_server_footprint_model():

@@ -353,3 +353,62 @@ def test_per_node_path_equals_grouped_with_leaf_targets(orc):
         else:  # without quadrupoles accepted cells are monopoles in the P2P list
             assert np.array_equal(pp + pc, gpp)
         assert np.sqrt(((a - g) ** 2).sum() / (g ** 2).sum()) < 1e-13
+
+
+def _cells_of(keys, depth):
+    """Translation (x, y, z) at `depth` of each 63-bit Morton key."""
+    def compact(k):
+        out = np.zeros_like(k)
+        for b in range(21):
+            out |= ((k >> np.uint64(3 * b)) & np.uint64(1)) << np.uint64(b)
+        return out
+    sh = (21 - depth).astype(np.uint64)
+    return np.stack([compact(keys >> np.uint64(a)) >> sh for a in range(3)], axis=1).astype(np.int64)
+
+
+PER_NODE_INPUTS = [("random_box", 3000, 8, 11), ("plummer", 20000, 8, 12), ("hernquist", 12000, 16, 13),
+                   ("plummer", 5000, 1, 14)]
+
+
+@pytest.mark.parametrize("kind,n,ml,seed", PER_NODE_INPUTS)
+def test_reference_tree_leaves_equal_flat_leaves(orc, ref, kind, n, ml, seed):
+    """The reference's per-node octree (tree.cc, built by the reference's own
+    code) has exactly the flat tree's leaves: same level, same cell, same
+    particles (no cell reaches the flat tree's depth cap on these inputs)."""
+    from bhgpu import ic
+
+    p = ic.random_box(n, seed, 10.0) if kind == "random_box" else getattr(ic, kind)(n, seed)
+    level, txyz, count, ids = ref.per_node_tree(p, ml).leaves()
+    e = orc.build(p, ml, 64).export()
+    nodes = e["nodes"]
+    lf = np.flatnonzero(nodes["leaf"] == 1)
+    lf = lf[np.argsort(nodes["particle_begin"][lf])]  # Morton order
+    assert len(lf) == len(level)
+    assert nodes["depth"][lf].max() < 21
+    assert np.array_equal(nodes["depth"][lf].astype(np.uint32), level)
+    assert np.array_equal(nodes["particle_count"][lf], count)
+    first = e["keys"][nodes["particle_begin"][lf]]
+    assert np.array_equal(_cells_of(first, nodes["depth"][lf].astype(np.int64)), txyz.astype(np.int64))
+    flat_ids = np.concatenate([np.sort(e["particles"]["id"][b:b + c])
+                               for b, c in zip(nodes["particle_begin"][lf], nodes["particle_count"][lf])])
+    assert np.array_equal(flat_ids, ids)
+
+
+@pytest.mark.parametrize("kind,n,ml,seed", PER_NODE_INPUTS)
+@pytest.mark.parametrize("theta,quad", [(0.5, True), (0.7, False), (0.3, True)])
+def test_per_node_pass_matches_reference_tree(orc, ref, kind, n, ml, seed, theta, quad):
+    """The oracle's per-node pass (on the flat tree) has, particle by
+    particle, the interaction lists of the dispatch recursion
+    (traversal_ttg.h:156-196) over the reference's own per-node tree and
+    multipoles (tree.cc, upward_ttg.h's moments), and its forces to 1e-12."""
+    from bhgpu import ic, layout
+
+    p = ic.random_box(n, seed, 10.0) if kind == "random_box" else getattr(ic, kind)(n, seed)
+    fp = layout.force_params(theta, 1e-4, 1.0, quad)
+    racc, rpp, rpc = ref.per_node_tree(p, ml).per_node_pass(fp)
+    t = orc.build(p, ml, 64)
+    oacc, opp, opc = t.per_node_pass(fp)
+    ids = t.export()["particles"]["id"]
+    assert np.array_equal(rpp[ids], opp) and np.array_equal(rpc[ids], opc)
+    err = np.sqrt(((racc[ids] - oacc) ** 2).sum() / (oacc ** 2).sum())
+    assert err < 1e-12, err
