@@ -43,6 +43,7 @@ SIGNATURES = {
     "bh_set_walk_mode": ([_P, _I], _I),
     "bh_set_graph_mode": ([_P, _I], _I),
     "bh_set_walk_tile": ([_P, _U32], _I),
+    "bh_debug_walk_stack_limit": ([_P, _U32], _I),
     "bh_graph_count": ([_P, ctypes.POINTER(_U32)], _I),
     "bh_max_particles": ([], _U64),
     "bh_step": ([_P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
@@ -223,6 +224,9 @@ class Context:
 
     def set_walk_tile(self, tile):
         self._check(self._lib.bh_set_walk_tile(self._h, tile))
+
+    def debug_walk_stack_limit(self, levels):
+        self._check(self._lib.bh_debug_walk_stack_limit(self._h, levels))
 
     def graph_count(self):
         k = _U32()

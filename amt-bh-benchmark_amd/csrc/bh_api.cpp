@@ -108,6 +108,7 @@ struct Ctx {
   int walk_mode = BH_WALK_GROUPED;
   int graph_mode = BH_GRAPH_OFF;
   uint32_t walk_tile = kGroupCap;
+  uint32_t stack_limit = kLevels;  // lowered only by bh_debug_walk_stack_limit
 
   DevBuf<bh_particle> pa, pb;    // particle ping-pong of the resident state
   int cur = 0;                   // which of pa/pb holds the resident state
@@ -566,6 +567,7 @@ int enqueue_walk(Ctx* c, const Plan& p, const Scratch& b, bh_particle* parts, do
   a.misc = b.misc;
   a.n_nodes = p.cap_nodes;
   a.n_parts = n;
+  a.stack_limit = c->stack_limit;
   a.parts = parts;
   a.acc = d_acc;
   a.n_pp = d_npp;
@@ -1010,6 +1012,18 @@ int bh_set_walk_tile(bh_ctx* h, uint32_t tile) {
     return BH_ERR_INVALID;
   }
   c->walk_tile = tile;
+  return BH_OK;
+}
+
+int bh_debug_walk_stack_limit(bh_ctx* h, uint32_t levels) {
+  Ctx* c = as(h);
+  if (!c) return BH_ERR_INVALID;
+  if (levels == 0 || levels > static_cast<uint32_t>(kLevels)) {
+    set_err(c, "walk stack limit %u outside 1..%d", levels, kLevels);
+    return BH_ERR_INVALID;
+  }
+  drop_graphs(c);  // the limit is a launch argument
+  c->stack_limit = levels;
   return BH_OK;
 }
 

@@ -123,6 +123,7 @@ struct WalkArgs {
   uint32_t* misc;          // the step's flags; unit count and queue heads
   uint32_t n_nodes;        // index clamps (the node capacity): a node or particle
   uint32_t n_parts;        // index never leaves its array, whatever the tree holds
+  uint32_t stack_limit;    // frames the walk may use (kLevels; lower only to test the incomplete-walk report)
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
   double* acc;             // [3n] G * acceleration (nullable)
   uint32_t* n_pp;          // [n] (nullable)

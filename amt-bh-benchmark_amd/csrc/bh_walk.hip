@@ -643,7 +643,7 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
     // a well-formed tree; if one does, the node is not opened and the step
     // reports an incomplete walk (the reference's missing > 0,
     // integrate_ttg.h:104-106).
-    const bool openable = !isleaf && lcnt != 0u && depth + 1 < kStack;
+    const bool openable = !isleaf && lcnt != 0u && depth + 1 < kStack && static_cast<uint32_t>(depth) + 1 < a.stack_limit;
     const bool openf = nv && !accept && !defer && !isleaf && openable;
     if (__ballot(nv && !accept && !defer && !isleaf && !openable)) {
       if (lane == 0) atomicOr(&a.misc[kMiscIncomplete], 1u);

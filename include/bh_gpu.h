@@ -127,6 +127,12 @@ int bh_set_graph_mode(bh_ctx* ctx, int mode);
 #define BH_MIN_WALK_TILE 16u
 #define BH_MAX_WALK_TILE 128u
 int bh_set_walk_tile(bh_ctx* ctx, uint32_t tile);
+/* Diagnostic (no reference counterpart): caps the walk's stack at `levels`
+ * frames (1..22; 22, the default, is every level a tree can have), so that
+ * a test can make the walk meet nodes it may not open and check the
+ * BH_ERR_INCOMPLETE report (the reference's missing > 0,
+ * integrate_ttg.h:104-106); the kernel still finishes normally. */
+int bh_debug_walk_stack_limit(bh_ctx* ctx, uint32_t levels);
 int bh_graph_count(bh_ctx* ctx, uint32_t* count);
 /* Limits: n < 2^30 particles, max_leaf <= 1024. */
 uint64_t bh_max_particles(void);

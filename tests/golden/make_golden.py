@@ -9,6 +9,8 @@ the reference's MAC and force functions.
 
     python tests/golden/make_golden.py
 """
+import hashlib
+import json
 import os
 import sys
 
@@ -44,8 +46,37 @@ CASES = {
 }
 
 
+# Larger inputs pinned by SHA-256 digests of the reference's outputs (the
+# arrays themselves would be megabytes): several backbone levels above the
+# 4096-particle subtrees, so the reference node numbering and group order of
+# flat_build_ttg.h:194-273 are pinned at depth. The input is regenerated
+# from the seeded in-repo generator, whose own digest is stored too.
+DIGEST_CASES = {
+    "plummer131k": (lambda: ic.plummer(131072, 11), "ic.plummer(131072, 11)", 16, 256, 32, 0.5, 1e-4),
+}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).view(np.uint8).tobytes()).hexdigest()
+
+
+def digest_fields(e, npp, npc):
+    return {k: sha(e[k]) for k in ("keys", "perm", "nodes", "center", "iab", "groups")} | {
+        "n_pp": sha(npp), "n_pc": sha(npc)}
+
+
 def main():
     R = oracle.Reference()
+    for name, (gen, desc, ml, gs, ts, theta, eps2) in DIGEST_CASES.items():
+        p = gen()
+        t = R.build(p, ml, gs)
+        e = t.export()
+        _, npp, npc = t.target_walk(theta, eps2, 1.0, True, ts)
+        d = {"input": desc, "n": len(p), "input_sha256": sha(p), "max_leaf": ml, "group_size": gs, "target_size": ts,
+             "theta": theta, "eps2": eps2, "n_nodes": len(e["nodes"]), "n_groups": len(e["groups"]),
+             "sha256": digest_fields(e, npp, npc)}
+        json.dump(d, open(os.path.join(HERE, f"{name}.json"), "w"), indent=1)
+        print(f"{name}: n={len(p)} nodes={len(e['nodes'])} groups={len(e['groups'])}")
     for name, (gen, ml, gs, ts, theta, eps2) in CASES.items():
         p = gen()
         t = R.build(p, ml, gs)

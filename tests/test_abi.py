@@ -68,3 +68,29 @@ def test_struct_layouts_match_c(tmp_path):
             ctypes.sizeof(bl.Completion), ctypes.sizeof(bl.ServerStats), layout.WALK_NODE.itemsize,
             ctypes.sizeof(bl.StepStats), bl.StepStats.step_ms.offset, bl.StepStats.reruns.offset]
     assert got == want
+
+
+def test_cpp_wrapper_maps_statuses(tmp_path):
+    """bh_gpu.hpp throws as the reference does: an incomplete walk as
+    std::logic_error (integrate_ttg.h:104-106), bad input as
+    std::invalid_argument, device failures as std::runtime_error."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    src = tmp_path / "w.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include "bh_gpu.hpp"\n'
+        "template <class E> int throws(int rc) {\n"
+        "  try { bh_gpu::throw_status(rc, \"x\"); } catch (const E&) { return 1; } catch (...) { return 0; }\n"
+        "  return 0;\n}\n"
+        "int main() {\n"
+        "  const int ok = throws<std::logic_error>(BH_ERR_INCOMPLETE) && throws<std::invalid_argument>(BH_ERR_INVALID) &&\n"
+        "                 throws<std::invalid_argument>(BH_ERR_NONFINITE) && throws<std::runtime_error>(BH_ERR_HIP) &&\n"
+        "                 throws<std::runtime_error>(BH_ERR_OOM);\n"
+        "  std::puts(ok ? \"ok\" : \"bad\");\n  return ok ? 0 : 1;\n}\n")
+    exe = tmp_path / "w"
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.dirname(HEADER), "-I/opt/rocm/include", "-o", str(exe), str(src)],
+                   check=True)
+    assert subprocess.run([str(exe)], capture_output=True, text=True).stdout.strip() == "ok"

@@ -360,3 +360,43 @@ def test_walk_tile_keeps_lists(gpu, orc, tile):
         gpu.set_precision(bl.BH_FP32)
     with pytest.raises(bl.BHError):
         gpu.set_walk_tile(8)
+
+
+def test_incomplete_walk_reported(gpu):
+    """The reference throws when a walk leaves interactions missing
+    (integrate_ttg.h:104-106). With the walk's stack capped below the tree's
+    depth (a diagnostic limit: the kernel still finishes normally) the walk
+    meets nodes it may not open, flags it, and the step and bh_accel return
+    BH_ERR_INCOMPLETE; with the full stack the same input steps normally."""
+    p = ic.plummer(20000, 93)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    try:
+        gpu.debug_walk_stack_limit(3)
+        with pytest.raises(bl.BHError) as e:
+            gpu.step(p, fp, 16, 256, 32, 1e-3, 1e-3)
+        assert e.value.code == 3 and "incomplete" in str(e.value)
+        gpu.build_tree(p, 16, 256)
+        with pytest.raises(bl.BHError) as e:
+            gpu.accel(fp, 32)
+        assert e.value.code == 3
+    finally:
+        gpu.debug_walk_stack_limit(22)
+    out, _ = gpu.step(p, fp, 16, 256, 32, 1e-3, 1e-3)
+    assert np.array_equal(np.sort(out["id"]), np.arange(len(p)))
+    with pytest.raises(bl.BHError):
+        gpu.debug_walk_stack_limit(0)
+
+
+@pytest.mark.parametrize("name", golden_io.DIGEST_NAMES)
+def test_reference_digests_at_depth(gpu, name):
+    """The GPU tree (reference node numbering, after the host renumbering of
+    several backbone levels) and the walk's per-particle list lengths hash to
+    the digests of the reference-compiled outputs at 131,072 particles."""
+    d = golden_io.load_digest(name)
+    p = golden_io.digest_input(d)
+    gpu.build_tree(p, d["max_leaf"], d["group_size"])
+    e = gpu.export()
+    got = {k: golden_io.sha(e[k]) for k in ("keys", "perm", "nodes", "center", "iab", "groups")}
+    _, npp, npc, _ = gpu.accel(layout.force_params(d["theta"], d["eps2"], 1.0, True), d["target_size"])
+    got |= {"n_pp": golden_io.sha(npp), "n_pc": golden_io.sha(npc)}
+    assert got == d["sha256"]

@@ -412,3 +412,22 @@ def test_per_node_pass_matches_reference_tree(orc, ref, kind, n, ml, seed, theta
     assert np.array_equal(rpp[ids], opp) and np.array_equal(rpc[ids], opc)
     err = np.sqrt(((racc[ids] - oacc) ** 2).sum() / (oacc ** 2).sum())
     assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("name", ["plummer131k"])
+def test_oracle_matches_reference_digests(orc, name):
+    """At 131,072 particles (several backbone levels above the 4096-particle
+    subtrees) the oracle's keys, permutation, node arrays, centres, second
+    moments, group order and per-particle list lengths hash to the digests of
+    the reference-compiled outputs."""
+    import golden_io
+    from bhgpu import layout
+
+    d = golden_io.load_digest(name)
+    p = golden_io.digest_input(d)
+    t = orc.build(p, d["max_leaf"], d["group_size"])
+    e = t.export()
+    _, npp, npc, _ = t.grouped_pass(layout.force_params(d["theta"], d["eps2"], 1.0, True), d["target_size"])
+    got = {k: golden_io.sha(e[k]) for k in ("keys", "perm", "nodes", "center", "iab", "groups")}
+    got |= {"n_pp": golden_io.sha(npp), "n_pc": golden_io.sha(npc)}
+    assert got == d["sha256"]
