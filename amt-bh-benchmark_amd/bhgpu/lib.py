@@ -48,6 +48,7 @@ SIGNATURES = {
     "bh_max_particles": ([], _U64),
     "bh_step": ([_P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_step_device": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
+    "bh_step_device_noalloc": ([_P, _P, _P, _U64, _P, _U32, _U32, _U32, _D, _D, _P], _I),
     "bh_step_batch": ([_P, _P, _P, _P, _U32, _P], _I),
     "bh_step_batch_host": ([_P, _P, _P, _U32, _P], _I),
     "bh_state_step_batch": ([_P, _P, _U32, _P], _I),
@@ -245,6 +246,13 @@ class Context:
         t = StepTiming()
         self._check(self._lib.bh_step_device(self._h, d_in, d_out, n, _ptr(_fp(fp)), max_leaf, group_size, target_size,
                                              kick_dt, drift_dt, ctypes.byref(t)))
+        return t
+
+    def step_device_noalloc(self, d_in, d_out, n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt):
+        """bh_step_device_noalloc: the step with its scratch in the library's fixed per-GPU array."""
+        t = StepTiming()
+        self._check(self._lib.bh_step_device_noalloc(self._h, d_in, d_out, n, _ptr(_fp(fp)), max_leaf, group_size,
+                                                     target_size, kick_dt, drift_dt, ctypes.byref(t)))
         return t
 
     def step_batch(self, p, systems):

@@ -36,6 +36,32 @@ static_assert(kGroupCap == BH_MAX_WALK_TILE && kMinTile == BH_MIN_WALK_TILE, "wa
 
 namespace {
 
+// The allocation-free entry point's working memory: one fixed array per GPU
+// (every call on that GPU shares it), reached through the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocBytes = size_t(BH_NOALLOC_SCRATCH_MIB) << 20;
+constexpr uint32_t kNoAllocSlots = 512;  // walk workgroups of the allocation-free step
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+hipError_t scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
@@ -109,6 +135,7 @@ struct Ctx {
   int graph_mode = BH_GRAPH_OFF;
   uint32_t walk_tile = kGroupCap;
   uint32_t stack_limit = kLevels;  // lowered only by bh_debug_walk_stack_limit
+  void* arena = nullptr;  // this device's allocation-free scratch
 
   DevBuf<bh_particle> pa, pb;    // particle ping-pong of the resident state
   int cur = 0;                   // which of pa/pb holds the resident state
@@ -254,8 +281,8 @@ int check_args(Ctx* c, uint64_t n, const bh_force_params* fp, uint32_t max_leaf,
   return BH_OK;
 }
 
-// Pinned staging of the system table; with no_grow it must already be
-// large enough.
+// Pinned staging of the system table; with no_grow (the allocation-free
+// step) it must already be large enough.
 int host_table_room(Ctx* c, size_t bytes, bool no_grow) {
   if (c->host_tables_cap >= bytes) return BH_OK;
   if (no_grow) {
@@ -489,6 +516,82 @@ int provision(Ctx* c, const Plan& p, Scratch* b) {
     s.deep_frames = p.deep ? c->deep_frames.p : nullptr;
   }
   return BH_OK;
+}
+
+// Cuts the plan's arrays out of a fixed block of device memory, in the
+// order and sizes of provision(), with the node capacity set to whatever is
+// left. Fails (returns false) when the per-particle arrays do not fit.
+bool carve(uint8_t* base, size_t bytes, Plan* p, Scratch* b) {
+  size_t off = 0;
+  bool ok = true;
+  auto take = [&](size_t sz) -> uint8_t* {
+    off = (off + 255) / 256 * 256;
+    uint8_t* q = base + off;
+    if (off + sz > bytes) ok = false;
+    off += sz;
+    return q;
+  };
+  const uint32_t n = p->n;
+  const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
+  Scratch& s = *b;
+  s = Scratch{};
+  s.keys_a = reinterpret_cast<uint64_t*>(take(8ull * n));
+  s.keys_b = reinterpret_cast<uint64_t*>(take(8ull * n));
+  s.vals_a = reinterpret_cast<uint32_t*>(take(4ull * n));
+  s.vals_b = reinterpret_cast<uint32_t*>(take(4ull * n));
+  s.hist = reinterpret_cast<uint32_t*>(take(4 * 10 * 256));
+  s.dbase = reinterpret_cast<uint32_t*>(take(4 * 10 * 256));
+  s.lookback = reinterpret_cast<uint32_t*>(take(4 * radix_lookback_words(n)));
+  s.lohi = reinterpret_cast<double*>(take(48ull * p->S));
+  s.partial = reinterpret_cast<double*>(take(48ull * p->n_chunks));
+  s.misc = reinterpret_cast<uint32_t*>(take(4 * kMiscWords));
+  s.totals = reinterpret_cast<unsigned long long*>(take(16));
+  s.posm = reinterpret_cast<double4*>(take(32ull * n));
+  s.L = take(n);
+  s.H = take(n);
+  s.tcounts = reinterpret_cast<uint32_t*>(take(4ull * tiles * kTreeCols));
+  s.tprefix = reinterpret_cast<uint32_t*>(take(4ull * tiles * kTreeCols));
+  s.sid = reinterpret_cast<uint16_t*>(take(2ull * n));
+  s.tables = take(p->table_bytes);
+  s.mark = reinterpret_cast<uint32_t*>(take(4ull * n));
+  s.targets = reinterpret_cast<uint32_t*>(take(4ull * n));
+  s.gmark = reinterpret_cast<uint32_t*>(take(4ull * n));
+  s.groups = reinterpret_cast<uint32_t*>(take(4ull * n));
+  s.gacc = reinterpret_cast<double*>(take(24ull * n));
+  s.defer = reinterpret_cast<uint32_t*>(take(4ull * p->slots * kDefer));
+  s.deep_frames = reinterpret_cast<uint32_t*>(take(4ull * p->slots * (kLevels - kShallowFrames) * kWave));
+  TreeDev& t = s.t;
+  t.level_off = reinterpret_cast<uint32_t*>(take(4 * (kLevels + 2)));
+  t.misc = s.misc;
+  if (!ok) return false;
+  // node arrays: 327 bytes per node, plus alignment of the 19 arrays
+  const size_t per_node = 4 * 6 + 3 + 32 + 64 + 8 + 24 + 48 + 64 + 48 + 4 * 3;
+  off = (off + 255) / 256 * 256;
+  const size_t left = bytes > off + 19 * 256 ? bytes - off - 19 * 256 : 0;
+  const uint64_t m = std::min<uint64_t>(left / per_node, 1ull << 30);
+  if (m < 64) return false;
+  t.begin = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.count = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.cs = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.first_child = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.parent = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.leaf_idx = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.depth = take(m);
+  t.leaf = take(m);
+  t.nchild = take(m);
+  t.ctr = reinterpret_cast<double4*>(take(32 * m));
+  t.rec = reinterpret_cast<NodeRec*>(take(64 * m));
+  t.mass = reinterpret_cast<double*>(take(8 * m));
+  t.com = reinterpret_cast<double*>(take(24 * m));
+  t.iab = reinterpret_cast<double*>(take(48 * m));
+  t.q64 = reinterpret_cast<double*>(take(64 * m));
+  t.nbox = reinterpret_cast<double*>(take(48 * m));
+  t.leaf_node = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.leaf_first = reinterpret_cast<uint32_t*>(take(4 * m));
+  s.unit_n = reinterpret_cast<uint32_t*>(take(4 * m));
+  t.cap = static_cast<uint32_t>(m);
+  p->cap_nodes = t.cap;
+  return ok;
 }
 
 struct Sorted {
@@ -859,17 +962,19 @@ int bh_ctx_create(bh_ctx** out, int device) {
   Ctx* c = new Ctx();
   c->device = device;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= kMaxDevices) {
     (void)hipGetLastError();
     delete c;
     return BH_ERR_HIP;
   }
-  // pinned staging sized so that small tables never have to grow
+  // pinned staging sized so that a one-system table of the allocation-free
+  // step never has to grow
   const size_t table0 = 64 << 10;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->host_misc), (kMiscWords + 4) * sizeof(uint32_t), hipHostMallocDefault) !=
           hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&c->host_tables), table0, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&c->host_tables), table0, hipHostMallocDefault) != hipSuccess ||
+      scratch_address(device, &c->arena) != hipSuccess) {
     (void)hipGetLastError();
     if (c->host_tables) (void)hipHostFree(c->host_tables);
     if (c->host_misc) (void)hipHostFree(c->host_misc);
@@ -1044,6 +1149,79 @@ int bh_step_device(bh_ctx* h, const bh_particle* d_in, bh_particle* d_out, uint6
     return BH_ERR_INVALID;
   }
   return step_device(c, d_in, d_out, n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing);
+}
+
+int bh_step_device_noalloc(bh_ctx* h, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
+                           uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt,
+                           double drift_dt, bh_step_timing* timing) {
+  Ctx* c = as(h);
+  if (!c || !fp || (n && (!d_in || !d_out))) return c ? (set_err(c, "null argument"), BH_ERR_INVALID) : BH_ERR_INVALID;
+  if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
+  if (n > BH_NOALLOC_MAX_N) {
+    set_err(c, "n = %llu exceeds the allocation-free step's limit of %u particles", (unsigned long long)n,
+            BH_NOALLOC_MAX_N);
+    return BH_ERR_TOO_LARGE;
+  }
+  if (overlap(d_in, d_out, n)) {
+    set_err(c, "d_in and d_out overlap");
+    return BH_ERR_INVALID;
+  }
+  TRY(check_args(c, n, fp, max_leaf, group_size));
+  if (n == 0) return BH_OK;
+  HIP_OK(c, hipSetDevice(c->device));
+  const HostSys one{n, *fp, max_leaf, group_size, target_size, kick_dt, drift_dt, 0};
+  Plan p;
+  TRY(plan_systems(c, &one, 1, true, &p));
+  // every level and the deep stack are planned from the start: the only
+  // thing that can outgrow the fixed scratch is the node count
+  p.walk = true;
+  p.depth_plan = kDepthCap;
+  p.deep = true;
+  p.slots = walk_slots(p.n, kNoAllocSlots);
+  Scratch b;
+  if (!carve(static_cast<uint8_t*>(c->arena), kNoAllocBytes, &p, &b)) {
+    set_err(c, "n = %llu does not fit the allocation-free step's %zu-byte scratch", (unsigned long long)n, kNoAllocBytes);
+    return BH_ERR_TOO_LARGE;
+  }
+  // the tree lives in the shared scratch: it is not this context's to export
+  c->has_tree = false;
+  c->have_stats = false;
+  c->reruns = 0;
+  Sorted sd;
+  HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
+  TRY(enqueue_build(c, p, b, d_in, d_out, true, &sd));
+  TRY(enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, true));
+  TRY(finish(c, b));
+  const uint32_t* m = c->host_misc;
+  if (m[kMiscNonfinite]) {
+    set_err(c, "non-finite position, velocity or mass in the input");
+    return BH_ERR_NONFINITE;
+  }
+  if (m[kMiscAbort]) {
+    uint64_t nodes = 0;
+    for (int d = 0; d < kLevels; ++d) nodes += m[kMiscLevels + d];
+    set_err(c, "the tree needs %llu nodes and %u leaves; the allocation-free scratch holds %u",
+            (unsigned long long)nodes, m[kMiscLevels + kLevels], p.cap_nodes);
+    return BH_ERR_TOO_LARGE;
+  }
+  c->n_targets = m[kMiscTargets];
+  c->n_groups = m[kMiscGroups];
+  std::memcpy(c->last_totals, m + kMiscWords, sizeof c->last_totals);
+  uint64_t nodes = 0;
+  for (int d = 0; d < kLevels; ++d) nodes += m[kMiscLevels + d];
+  c->n_nodes = static_cast<uint32_t>(nodes);
+  c->n_leaves = m[kMiscLevels + kLevels];
+  c->have_stats = true;
+  c->last_graph = false;
+  if (timing) {
+    timing->tree_build_ms = elapsed(c->ev[0], c->ev[2]);
+    timing->traversal_ms = elapsed(c->ev[2], c->ev[5]);
+  }
+  if (m[kMiscIncomplete]) {
+    set_err(c, "incomplete tree: the walk met an interior node it could not open");
+    return BH_ERR_INCOMPLETE;
+  }
+  return BH_OK;
 }
 
 int bh_step(bh_ctx* h, bh_particle* particles, uint64_t n, const bh_force_params* fp, uint32_t max_leaf,

@@ -155,6 +155,24 @@ int bh_step_device(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, uin
                    uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
                    bh_step_timing* timing);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). bh_step_device for n <= BH_NOALLOC_MAX_N particles with every
+ * intermediate array (keys, tree, walk scratch) cut from one fixed
+ * BH_NOALLOC_SCRATCH_MIB MiB device array that the library holds per GPU
+ * from its first context on that GPU until the process ends; the call
+ * allocates nothing. Same kernels, same results as bh_step_device, byte for
+ * byte. Larger n, or a tree with more nodes than the fixed array holds,
+ * returns BH_ERR_TOO_LARGE. Every call on one GPU uses that GPU's single
+ * array: calls that overlap on one GPU (other threads or contexts) race and
+ * must be serialised by the caller. The call does not leave a tree for
+ * bh_tree_export / bh_accel. */
+#define BH_NOALLOC_MAX_N 65536u
+#define BH_NOALLOC_SCRATCH_MIB 64u
+int bh_step_device_noalloc(bh_ctx* ctx, const bh_particle* d_in, bh_particle* d_out, uint64_t n,
+                           const bh_force_params* fp, uint32_t max_leaf, uint32_t group_size, uint32_t target_size,
+                           double kick_dt, double drift_dt, bh_step_timing* timing);
+
 /* One independent system of a batch: n particles (consecutive in the batch
  * arrays, in table order) and its own step_particles arguments. */
 typedef struct bh_system {
