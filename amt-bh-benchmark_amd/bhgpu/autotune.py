@@ -8,13 +8,17 @@ share walks, i.e. the time. For each size class and opening angle the tuner
 measures, on a Plummer and a Hernquist system of that size (eps = 0.01):
 
 - the rms relative acceleration error of every (leaf, group, target) shape
-  against the FP64 direct sum (seeded subset, worse of the two systems);
-- the step time (median of a few steps, summed over the two systems) of
-  every walk tile for each shape within the error bound.
+  against the FP64 direct sum (seeded subset, worst of two Plummer and two
+  Hernquist systems);
+- the step time (median of a few steps, summed over a Plummer and a
+  Hernquist system) of every walk tile for each shape within the target.
 
-The bound is a hard constraint: the fastest (shape, tile) within it wins.
-Only when no shape reaches it is the most accurate shape kept, with the
-class marked within_bound = false and its error floor recorded.
+The error is a hard constraint with a margin: the fastest (shape, tile)
+whose error is within MARGIN times the bound wins, so that other seeds and
+nearby sizes, whose errors scatter by a fifth or more, stay within the
+bound. When no shape reaches the target the most accurate shape is kept;
+the class records whether that shape is within the bound itself
+(within_bound) and the error floor of all shapes.
 
 The table ((size class, theta) -> shapes and the measurements behind the
 choice) is written as JSON next to this module; `shapes_for(n, theta)`
@@ -41,6 +45,7 @@ LEAVES = (8, 16, 32, 64)
 GROUPS = (64, 128, 256)
 TARGETS = (16, 32, 64, 128, 256)
 TILES = (32, 64, 128)
+MARGIN = 0.8
 
 _cache = {}
 
@@ -54,13 +59,21 @@ def load_table(path=TABLE):
     return _cache[key]
 
 
+def class_for(n, theta=0.5, path=TABLE):
+    """The table's entry for a system of n particles (nearest size class in
+    log N, then nearest theta), or None without a table."""
+    t = load_table(path)
+    if not t or not t.get("classes"):
+        return None
+    return min(t["classes"], key=lambda c: (abs(np.log(max(n, 1)) - np.log(c["n"])), abs(theta - c.get("theta", theta))))
+
+
 def shapes_for(n, theta=0.5, path=TABLE):
     """(max_leaf, group_size, target_size, walk_tile) for a system of n
     particles; tables without a tile give the largest tile."""
-    t = load_table(path)
-    if not t or not t.get("classes"):
+    best = class_for(n, theta, path)
+    if best is None:
         return DEFAULT
-    best = min(t["classes"], key=lambda c: (abs(np.log(max(n, 1)) - np.log(c["n"])), abs(theta - c.get("theta", theta))))
     sh = tuple(best["shapes"])
     return sh if len(sh) == 4 else sh + (MAX_TILE,)
 
@@ -95,35 +108,45 @@ def _time(ctx, p, fp, shapes, reps=3):
     return float(np.median(ts_ms))
 
 
-def tune(ctx, sizes, thetas=(0.4, 0.5, 0.7), eps=0.01, bound=1e-3, log=print):
+def tune(ctx, sizes, thetas=(0.4, 0.5, 0.7), eps=0.01, bound=1e-3, margin=MARGIN, log=print):
     classes = []
     for n in sizes:
-        systems = [ic.plummer(n, 31), ic.hernquist(n, 32)]
+        systems = [ic.plummer(n, 31), ic.hernquist(n, 32), ic.plummer(n, 33), ic.hernquist(n, 34)]
         for theta in thetas:
             fp = layout.force_params(theta, eps * eps, 1.0, True)
             shapes = [(ml, gs, ts) for ml in LEAVES for gs in GROUPS for ts in TARGETS if ts <= gs]
             errs = {sh: max(_accuracy(ctx, p, fp, sh) for p in systems) for sh in shapes}
-            ok = [sh for sh in shapes if errs[sh] <= bound]
+            ok = [sh for sh in shapes if errs[sh] <= margin * bound]
             timed = ok if ok else [min(shapes, key=lambda sh: errs[sh])]
             rows = []
             for sh in timed:
                 for tile in TILES:
-                    ms = sum(_time(ctx, p, fp, sh + (tile,)) for p in systems)
+                    ms = sum(_time(ctx, p, fp, sh + (tile,)) for p in systems[:2])
                     rows.append({"shapes": list(sh) + [tile], "ms": ms, "rms_rel": errs[sh]})
             best = min(rows, key=lambda r: r["ms"])
             dflt_err = errs.get(DEFAULT[:3])
             log(f"n={n} theta={theta}: best {best['shapes']} {best['ms']:.3f} ms (rms {best['rms_rel']:.2e}, "
-                f"{len(ok)} of {len(shapes)} shapes within {bound:g}; error floor {min(errs.values()):.2e})")
+                f"{len(ok)} of {len(shapes)} shapes within {margin:g} x {bound:g}; "
+                f"error floor {min(errs.values()):.2e})")
             classes.append({"n": int(n), "theta": theta, "shapes": best["shapes"], "ms": best["ms"],
-                            "rms_rel": best["rms_rel"], "within_bound": bool(ok),
+                            "rms_rel": best["rms_rel"], "within_target": bool(ok),
+                            "within_bound": best["rms_rel"] <= bound,
                             "rms_floor": min(errs.values()), "default_rms_rel": dflt_err,
                             "shape_errors": [{"shapes": list(sh), "rms_rel": e} for sh, e in errs.items()],
                             "timings": rows})
-    return {"eps": eps, "bound": bound,
-            "measure": "rms: worse of a Plummer and a Hernquist system (FP64 direct sum, 256 particles); "
-                       "time: median of 3 steps, summed over the two systems, for every walk tile of every "
-                       "shape within the bound",
+    return {"eps": eps, "bound": bound, "margin": margin,
+            "measure": "rms: worst of two Plummer and two Hernquist systems (FP64 direct sum, 256 particles "
+                       "each); time: median of 3 steps, summed over a Plummer and a Hernquist system, for "
+                       "every walk tile of every shape within margin x bound",
             "classes": classes}
+
+
+def dump(table, f):
+    """The table as JSON with one size class per line."""
+    head = {k: v for k, v in table.items() if k != "classes"}
+    f.write(json.dumps(head)[:-1] + ', "classes": [\n')
+    f.write(",\n".join(json.dumps(c, separators=(",", ":")) for c in table["classes"]))
+    f.write("\n]}\n")
 
 
 def main(argv=None):
@@ -137,7 +160,7 @@ def main(argv=None):
                  log=lambda *a: print(*a, file=sys.stderr, flush=True))
     ctx.close()
     with open(args.out, "w") as f:
-        json.dump(table, f, indent=1)
+        dump(table, f)
     print(json.dumps({"classes": [{k: c[k] for k in ("n", "theta", "shapes", "ms", "rms_rel", "within_bound", "rms_floor")}
                                   for c in table["classes"]]}))
     return 0

@@ -1098,6 +1098,19 @@ __global__ __launch_bounds__(256) void k_direct_reduce(const double* __restrict_
   acc[3 * uint64_t(k) + 2] = G * a2;
 }
 
+// e + q.w / |t - q| in FP64. The hardware reciprocal square root is refined
+// by the second-order step y (1 + r/2 + 3r^2/8), r = 1 - x y^2 (the Taylor
+// series of (1 - r)^(-1/2)), which leaves a relative error of order r^3: far
+// below FP64 rounding for the hardware estimate's r.
+__device__ __forceinline__ double pot_term(const double4& t, const double4& q, double e) {
+  const double dx = t.x - q.x, dy = t.y - q.y, dz = t.z - q.z;
+  const double x = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  const double y = __builtin_amdgcn_rsq(x);
+  const double r = __builtin_fma(-(x * y), y, 1.0);
+  const double p = __builtin_fma(r, __builtin_fma(r, 0.375, 0.5), 1.0);
+  return __builtin_fma(q.w * y, p, e);
+}
+
 // potential_energy (integrate.h:35-43) over unordered pairs: particle i
 // adds m_j / r_ij for the partners j = i + d (mod n), d = 1 .. floor(n / 2),
 // which counts every pair once (in an even-sized set the pair n / 2 apart is
@@ -1122,7 +1135,10 @@ __global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ s
   double4 t = make_double4(0, 0, 0, 0);
   if (ok) t = src[i];
   double e = 0.0;
-  // partners of the block: positions i0 + d_lo .. i0 + 255 + d_hi - 1 (unwrapped)
+  // partners of the block: positions i0 + d_lo .. i0 + 255 + d_hi - 1
+  // (unwrapped). Every lane reads the same partner at the same time (an LDS
+  // broadcast); a tile where some lane's window starts or ends is masked
+  // per lane, the tiles in between (nearly all) are not.
   const uint64_t w_lo = uint64_t(i0) + d_lo, w_hi = uint64_t(i0) + 255 + d_hi;
   for (uint64_t jb = w_lo; jb < w_hi; jb += 256) {
     __syncthreads();
@@ -1131,16 +1147,22 @@ __global__ __launch_bounds__(256) void k_potential(const double4* __restrict__ s
       if (j < w_hi) s_src[threadIdx.x] = src[j % n];
     }
     __syncthreads();
-    if (!ok) continue;
-    // unwrapped partner position of offset d is i + d
+    const int cnt = static_cast<int>(w_hi - jb < 256 ? w_hi - jb : 256);
+    // this lane's partners in the tile: [lo, hi) (unwrapped partner position
+    // of offset d is i + d)
     const int64_t k_lo = static_cast<int64_t>(uint64_t(i) + d_lo) - static_cast<int64_t>(jb);
     const int64_t k_hi = static_cast<int64_t>(uint64_t(i) + my_hi) - static_cast<int64_t>(jb);
-    const int lo = static_cast<int>(k_lo < 0 ? 0 : (k_lo > 256 ? 256 : k_lo));
-    const int hi = static_cast<int>(k_hi < 0 ? 0 : (k_hi > 256 ? 256 : k_hi));
-    for (int k = lo; k < hi; ++k) {
-      const double4 q = s_src[k];
-      const double dx = t.x - q.x, dy = t.y - q.y, dz = t.z - q.z;
-      e += q.w * rsqrt(dx * dx + dy * dy + dz * dz);
+    const int lo = ok ? static_cast<int>(k_lo < 0 ? 0 : (k_lo > 256 ? 256 : k_lo)) : 0;
+    const int hi = ok ? static_cast<int>(k_hi < 0 ? 0 : (k_hi > 256 ? 256 : k_hi)) : 0;
+    const bool whole = __syncthreads_and(lo == 0 && hi >= cnt);
+    if (whole) {
+#pragma unroll 4
+      for (int k = 0; k < cnt; ++k) e = pot_term(t, s_src[k], e);
+    } else {
+      for (int k = 0; k < cnt; ++k) {
+        const double with = pot_term(t, s_src[k], e);
+        e = (k >= lo && k < hi) ? with : e;
+      }
     }
   }
   double v = ok ? t.w * e : 0.0;
