@@ -23,7 +23,7 @@ namespace bhk {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kItems = 16;  // kSortTile / kBlock
+constexpr int kItems = static_cast<int>(kSortTile) / kBlock;
 
 __device__ inline uint64_t lanemask_lt() {
   const int lane = __lane_id();
@@ -214,9 +214,9 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
   __shared__ uint32_t s_hist[4][256];   // per-wave digit counters, then per-wave exclusive bases
   __shared__ uint32_t s_lexcl[256];     // tile-local exclusive digit offsets
   __shared__ uint32_t s_adj[256];       // global position - local position, per digit
-  __shared__ uint64_t s_k[kSortTile];
-  __shared__ uint32_t s_v[kSortTile];
+  __shared__ uint64_t s_k[kSortTile];   // the tile in digit order: keys, then (reused) payloads
   __shared__ uint8_t s_d[kSortTile];
+  uint32_t* s_v = reinterpret_cast<uint32_t*>(s_k);
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_tile;
 
@@ -295,6 +295,8 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
   }
   s_adj[t] = dbase[t] + excl - lex;
   __syncthreads();
+  // keys, then payloads, through the same LDS tile in digit order, so that
+  // the stores coalesce (one tile buffer keeps three workgroups per CU)
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
@@ -302,16 +304,30 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
       const uint32_t d = dg[j];
       const uint32_t pos = s_lexcl[d] + s_hist[w][d] + r[j];
       s_k[pos] = k[j];
-      s_v[pos] = v[j];
       s_d[pos] = static_cast<uint8_t>(d);
+      r[j] = pos;
     }
   }
   __syncthreads();
   const uint32_t tn = n - base < kSortTile ? n - base : kSortTile;
-  for (uint32_t e = tid; e < tn; e += kBlock) {
-    const uint32_t g = s_adj[s_d[e]] + e;
-    kout[g] = s_k[e];
-    vout[g] = s_v[e];
+  uint32_t g[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t e = tid + j * kBlock;
+    g[j] = e < tn ? s_adj[s_d[e]] + e : 0u;
+    if (e < tn) kout[g[j]] = s_k[e];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t idx = base + w * (kSortTile / 4) + j * 64 + lane;
+    if (idx < n) s_v[r[j]] = v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const uint32_t e = tid + j * kBlock;
+    if (e < tn) vout[g[j]] = s_v[e];
   }
 }
 

@@ -624,7 +624,8 @@ __device__ bool traverse(const WalkArgs& a, Lds& L, uint32_t* gfr, const Box6& b
     const double near2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
     const bool accept = theta2 * near2 > mac.w;
     if constexpr (kGroup) {
-      if (!accept) {
+      // no deferred list (a target's walk through this evaluator): accept or open
+      if (!accept && dl != nullptr) {
         const double fx0 = fmax(mac.x - blo0, bhi0 - mac.x);
         const double fx1 = fmax(mac.y - blo1, bhi1 - mac.y);
         const double fx2 = fmax(mac.z - blo2, bhi2 - mac.z);
@@ -821,10 +822,51 @@ __device__ inline Box6 node_box(const WalkArgs& a, uint32_t node) {
   return box;
 }
 
-// One target (node tnode): the walk over `start` (the group's deferred
-// nodes) or from the root, for each chunk of <= 64 of its particles; then
-// the slices' sums, plus the group's share (gacc, counts g_pp / g_pc) when
-// given, times G, to the outputs and the fused kick/drift.
+// The end of one target particle's walk: its FP64 sums plus the group's
+// share (gacc, when given), times G, to the outputs (grouped_ttg.h:63-66,
+// the update callback) and the fused kick/drift.
+__device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax, double ay, double az,
+                                       const double* gacc, double sys_G, double kick_dt, double drift_dt,
+                                       uint32_t n_pp, uint32_t n_pc) {
+  if (gacc) {
+    ax += gacc[3 * uint64_t(k3) + 0];
+    ay += gacc[3 * uint64_t(k3) + 1];
+    az += gacc[3 * uint64_t(k3) + 2];
+  }
+  const double gx = sys_G * ax, gy = sys_G * ay, gz = sys_G * az;
+  if (a.acc) {
+    a.acc[3 * uint64_t(k3) + 0] = gx;
+    a.acc[3 * uint64_t(k3) + 1] = gy;
+    a.acc[3 * uint64_t(k3) + 2] = gz;
+  }
+  if (a.n_pp) a.n_pp[k3] = n_pp;
+  if (a.n_pc) a.n_pc[k3] = n_pc;
+  if (a.parts) {
+    // kick_one / drift_one (integrate.h:15-16): v += a*dt; x += v*dt
+    bh_particle& pt = a.parts[k3];
+    const double vx = pt.vel[0] + gx * kick_dt;
+    const double vy = pt.vel[1] + gy * kick_dt;
+    const double vz = pt.vel[2] + gz * kick_dt;
+    pt.vel[0] = vx;
+    pt.vel[1] = vy;
+    pt.vel[2] = vz;
+    if (drift_dt != 0.0) {
+      pt.pos[0] = pt.pos[0] + vx * drift_dt;
+      pt.pos[1] = pt.pos[1] + vy * drift_dt;
+      pt.pos[2] = pt.pos[2] + vz * drift_dt;
+    }
+  }
+}
+
+// The largest target the group evaluator's layouts take (its slots hold
+// up to 64 particles in every layout); smaller targets use lane = (particle,
+// slice), larger ones walk in chunks of 64 particles.
+constexpr uint32_t kWideMin = 33, kWideMax = kWave;
+
+// One target (node tnode) of at most 32 or more than 64 particles: the
+// walk over `start` (the group's deferred nodes) or from the root, for each
+// chunk of <= 64 of its particles; then the slices' sums, plus the group's
+// share (gacc, counts g_pp / g_pc) when given.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t tnode, const uint32_t* start,
                             uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc,
@@ -834,7 +876,6 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
   const uint32_t T = uniform(a.count[tnode]);
   const uint32_t sys_id = uniform(a.sid[tb]);
   const SysDev& sp = a.sys[sys_id];
-  const double sys_G = sp.G, kick_dt = sp.kick_dt, drift_dt = sp.drift_dt;
   const bool use_quad = sp.use_quad != 0;
   const double4 tcv = a.ctr[tnode];
   const double tcx = tcv.x, tcy = tcv.y, tcz = tcv.z;
@@ -880,35 +921,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
         ay += red[3 * src + 1];
         az += red[3 * src + 2];
       }
-      const uint32_t k3 = tb + c0 + lane;
-      if (gacc) {
-        ax += gacc[3 * uint64_t(k3) + 0];
-        ay += gacc[3 * uint64_t(k3) + 1];
-        az += gacc[3 * uint64_t(k3) + 2];
-      }
-      const double gx = sys_G * ax, gy = sys_G * ay, gz = sys_G * az;
-      if (a.acc) {
-        a.acc[3 * uint64_t(k3) + 0] = gx;
-        a.acc[3 * uint64_t(k3) + 1] = gy;
-        a.acc[3 * uint64_t(k3) + 2] = gz;
-      }
-      if (a.n_pp) a.n_pp[k3] = n_pp;
-      if (a.n_pc) a.n_pc[k3] = n_pc;
-      if (a.parts) {
-        // kick_one / drift_one (integrate.h:15-16): v += a*dt; x += v*dt
-        bh_particle& pt = a.parts[k3];
-        const double vx = pt.vel[0] + gx * kick_dt;
-        const double vy = pt.vel[1] + gy * kick_dt;
-        const double vz = pt.vel[2] + gz * kick_dt;
-        pt.vel[0] = vx;
-        pt.vel[1] = vy;
-        pt.vel[2] = vz;
-        if (drift_dt != 0.0) {
-          pt.pos[0] = pt.pos[0] + vx * drift_dt;
-          pt.pos[1] = pt.pos[1] + vy * drift_dt;
-          pt.pos[2] = pt.pos[2] + vz * drift_dt;
-        }
-      }
+      finish_particle(a, tb + c0 + lane, ax, ay, az, gacc, sp.G, sp.kick_dt, sp.drift_dt, n_pp, n_pc);
     }
     tot_pp += static_cast<unsigned long long>(n_pp) * tc;
     tot_pc += static_cast<unsigned long long>(n_pc) * tc;
@@ -923,6 +936,13 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
 // every target inside shares - evaluating the shared list for all the
 // unit's particles and keeping the deferred nodes; then each target of the
 // unit walks only those.
+//
+// The unit's work is a sequence of jobs: the group's shared walk (if any),
+// then its targets in particle order. The group's walk and the targets of
+// 33-64 particles use the group evaluator (its layouts keep the lanes
+// busy for those sizes) through one call of its traversal; the other
+// targets use the target evaluator. Deferral is off for target jobs
+// (no deferred list): their decisions are accept or open.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g) {
   const int lane = __lane_id();
@@ -935,35 +955,84 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g)
   const SysDev& sp = a.sys[sys_id];
   const bool gleaf = (a.rec[g].count >> 31) != 0;
   uint32_t* gfr = Lds::kDeepFrames ? a.deep_frames + uint64_t(slot) * (kLevels - Lds::kFrames) * kWave : nullptr;
-  if (!packed && (gleaf || G <= sp.tsm || G > kGroupCap)) {
-    unsigned long long tot_pp = 0, tot_pc = 0;
-    walk_target<Real, kMask>(a, L, gfr, g, nullptr, 0, nullptr, 0, 0, tot_pp, tot_pc);
-    add_totals(a, tot_pp, tot_pc);
-    return;
-  }
-  // a packed unit's box and the origin of its relative coordinates (the
-  // box centre); a group's: its node box and centre
-  const Box6 box = packed ? particle_box(a, gb, G) : node_box(a, g);
-  const double4 gcv = packed ? make_double4(0.5 * (box.v[0] + box.v[3]), 0.5 * (box.v[1] + box.v[4]),
-                                            0.5 * (box.v[2] + box.v[5]), 0.0)
-                             : a.ctr[g];
-  GroupEval<Real, kMask> ev;
-  ev.setup(G, lane);
-  ev.eps2 = static_cast<Real>(sp.eps2);
-#pragma unroll
-  for (int k = 0; k < kGroupChunks; ++k) {
-    const double4 q = a.posm[gb + ev.particle(k, lane, G)];
-    ev.gx[k] = static_cast<Real>(q.x - gcv.x);
-    ev.gy[k] = static_cast<Real>(q.y - gcv.y);
-    ev.gz[k] = static_cast<Real>(q.z - gcv.z);
-  }
+  const bool single = !packed && (gleaf || G <= sp.tsm || G > kGroupCap);
   uint32_t* dl = a.defer + uint64_t(slot) * kDefer;
   uint32_t g_pp = 0, g_pc = 0, n_def = 0;
-  const bool ok = traverse<Real, kMask, true>(a, L, gfr, box, gcv.x, gcv.y, gcv.z, sys_id, nullptr, 0,
-                                              sp.use_quad != 0, sp.theta2, ev, g_pp, g_pc, dl, n_def);
-  if (ok) {
+  bool ok = false;  // the group's shared walk kept all its deferred nodes
+  unsigned long long tot_pp = 0, tot_pc = 0;
+  bool group_job = !single, single_job = single;
+  uint32_t c = 0;     // next 64-particle chunk of the unit to scan for targets
+  uint64_t bits = 0;  // targets of the scanned chunk not yet walked
+  uint32_t m = 0;     // this lane's mark in the scanned chunk
+  for (;;) {
+    // the next job: the group's walk, or a target node
+    bool is_group = false;
+    uint32_t tnode = g;
+    if (group_job) {
+      is_group = true;
+      group_job = false;
+    } else if (single_job) {
+      single_job = false;
+    } else {
+      if (single) break;
+      while (!bits && c < G) {
+        const uint32_t i = c + lane;
+        m = i < G ? a.tmark[gb + i] : 0u;
+        bits = __ballot(m != 0);
+        c += kWave;
+      }
+      if (!bits) break;
+      const int j = __ffsll(static_cast<long long>(bits)) - 1;
+      bits &= bits - 1;
+      tnode = uniform(__builtin_amdgcn_readlane(m, j)) - 1u;
+    }
+    const uint32_t* start = ok ? dl : nullptr;
+    const uint32_t n_start = ok ? n_def : 0u;
+    const double* gacc = ok ? a.gacc : nullptr;
+    const uint32_t jpp = ok ? g_pp : 0u, jpc = ok ? g_pc : 0u;
+    const uint32_t T = is_group ? 0u : uniform(a.count[tnode]);
+    if (!is_group && !(T >= kWideMin && T <= kWideMax)) {
+      walk_target<Real, kMask>(a, L, gfr, tnode, start, n_start, gacc, jpp, jpc, tot_pp, tot_pc);
+      continue;
+    }
+    // group evaluator: the group's shared walk, or one target of 33-64 particles
+    const uint32_t b0 = is_group ? gb : uniform(a.begin[tnode]);
+    const uint32_t size = is_group ? G : T;
+    // a packed unit's box and the origin of its relative coordinates (the
+    // box centre); a group's or target's: its node box and centre
+    const bool pbox = is_group && packed;
+    const Box6 box = pbox ? particle_box(a, gb, G) : node_box(a, tnode);
+    const double4 ccv = pbox ? make_double4(0.5 * (box.v[0] + box.v[3]), 0.5 * (box.v[1] + box.v[4]),
+                                            0.5 * (box.v[2] + box.v[5]), 0.0)
+                             : a.ctr[tnode];
+    GroupEval<Real, kMask> ev;
+    ev.setup(size, lane);
+    ev.eps2 = static_cast<Real>(sp.eps2);
+#pragma unroll
+    for (int k = 0; k < kGroupChunks; ++k) {
+      const double4 q = a.posm[b0 + ev.particle(k, lane, size)];
+      ev.gx[k] = static_cast<Real>(q.x - ccv.x);
+      ev.gy[k] = static_cast<Real>(q.y - ccv.y);
+      ev.gz[k] = static_cast<Real>(q.z - ccv.z);
+    }
+    uint32_t n_pp = 0, n_pc = 0, j_def = 0;
+    const bool walked = traverse<Real, kMask, true>(a, L, gfr, box, ccv.x, ccv.y, ccv.z, sys_id,
+                                                    is_group ? nullptr : start, is_group ? 0u : n_start,
+                                                    sp.use_quad != 0, sp.theta2, ev, n_pp, n_pc,
+                                                    is_group ? dl : nullptr, j_def);
+    if (is_group) {
+      ok = walked;
+      g_pp = n_pp;
+      g_pc = n_pc;
+      n_def = j_def;
+      if (!ok) continue;  // the targets walk from the root
+    } else {
+      n_pp += jpp;
+      n_pc += jpc;
+    }
     // per-particle sums: each slot's slices added in FP64 through LDS (the
-    // P2C ring is free after the walk), then stored for the targets
+    // P2C ring is free after the walk); the group's are stored for its
+    // targets, a target's finish its particles
     double* red = reinterpret_cast<double*>(L.pc.a);  // 3 x 64 doubles
     const int used = ev.nc == 2 || ev.c1 > 0 ? 2 : 1;
 #pragma unroll
@@ -975,7 +1044,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g)
         red[3 * lane + 2] = ev.acc[k].sz();
         __builtin_amdgcn_wave_barrier();
         const uint32_t i = ev.base[k] + static_cast<uint32_t>(lane);
-        if (static_cast<uint32_t>(lane) < P && i < G) {
+        if (static_cast<uint32_t>(lane) < P && i < size) {
           double ax = 0.0, ay = 0.0, az = 0.0;
           for (uint32_t s2 = 0; s2 < S; ++s2) {
             const uint32_t src = s2 * P + static_cast<uint32_t>(lane);
@@ -983,30 +1052,23 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g)
             ay += red[3 * src + 1];
             az += red[3 * src + 2];
           }
-          a.gacc[3 * uint64_t(gb + i) + 0] = ax;
-          a.gacc[3 * uint64_t(gb + i) + 1] = ay;
-          a.gacc[3 * uint64_t(gb + i) + 2] = az;
+          if (is_group) {
+            a.gacc[3 * uint64_t(gb + i) + 0] = ax;
+            a.gacc[3 * uint64_t(gb + i) + 1] = ay;
+            a.gacc[3 * uint64_t(gb + i) + 2] = az;
+          } else {
+            finish_particle(a, b0 + i, ax, ay, az, gacc, sp.G, sp.kick_dt, sp.drift_dt, n_pp, n_pc);
+          }
         }
         __builtin_amdgcn_wave_barrier();
       }
     }
-    // the deferred list and the group sums are read back by other lanes
-    __threadfence_block();
-  }
-  // the group's targets, in particle order (marked at their first particle)
-  unsigned long long tot_pp = 0, tot_pc = 0;
-  for (uint32_t c = 0; c < G; c += kWave) {
-    const uint32_t i = c + lane;
-    const uint32_t m = i < G ? a.tmark[gb + i] : 0u;
-    uint64_t bits = __ballot(m != 0);
-    while (bits) {
-      const int j = __ffsll(static_cast<long long>(bits)) - 1;
-      bits &= bits - 1;
-      const uint32_t tnode = uniform(__builtin_amdgcn_readlane(m, j)) - 1u;
-      if (ok)
-        walk_target<Real, kMask>(a, L, gfr, tnode, dl, n_def, a.gacc, g_pp, g_pc, tot_pp, tot_pc);
-      else
-        walk_target<Real, kMask>(a, L, gfr, tnode, nullptr, 0, nullptr, 0, 0, tot_pp, tot_pc);
+    if (is_group) {
+      // the deferred list and the group sums are read back by other lanes
+      __threadfence_block();
+    } else {
+      tot_pp += static_cast<unsigned long long>(n_pp) * size;
+      tot_pc += static_cast<unsigned long long>(n_pc) * size;
     }
   }
   add_totals(a, tot_pp, tot_pc);

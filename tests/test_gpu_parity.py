@@ -101,6 +101,34 @@ def test_walk_parity(gpu, orc, n, ml, gs, ts, theta, eps2, quad):
 
 
 @pytest.mark.parametrize("mode", [bl.BH_WALK_GROUPED, bl.BH_WALK_PER_TARGET])
+def test_wide_targets(gpu, orc, mode):
+    """Targets of 33-64 particles go through the group evaluator's layouts
+    (two slots for 33-42, 32 + the rest for 43-53, one slot for 54-64);
+    smaller and larger ones through the target evaluator. Target size 64
+    over leaves of 16 gives all of them; lists exact, sums within
+    tolerance, in both walk schedules."""
+    p = ic.plummer(40000, 23)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    ot = orc.build(p, 16, 256)
+    sizes = ot.export()["nodes"]["particle_count"][ot.targets(64)]
+    for lo, hi in ((1, 32), (33, 42), (43, 53), (54, 64)):
+        assert ((sizes >= lo) & (sizes <= hi)).any(), (lo, hi)
+    oa, opp, opc, otot = ot.grouped_pass(fp, 64)
+    gpu.set_walk_mode(mode)
+    try:
+        gpu.build_tree(p, 16, 256)
+        for prec in (bl.BH_FP64, bl.BH_FP32):
+            print(f"wide targets mode={mode} prec={prec}", flush=True)
+            gpu.set_precision(prec)
+            ga, gpp, gpc, gtot = gpu.accel(fp, 64)
+            assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc) and np.array_equal(gtot, otot)
+            assert rel_err(ga, oa) < TOL[prec]
+    finally:
+        gpu.set_walk_mode(bl.BH_WALK_GROUPED)
+        gpu.set_precision(bl.BH_FP32)
+
+
+@pytest.mark.parametrize("mode", [bl.BH_WALK_GROUPED, bl.BH_WALK_PER_TARGET])
 @pytest.mark.parametrize("theta,kind", [(0.1, "plummer"), (0.5, "plummer"), (0.8, "hernquist"), (0.5, "hernquist")])
 def test_walk_modes_clustered(gpu, orc, mode, theta, kind):
     # theta 0.1 defers more nodes than a group may hold: its groups fall back
