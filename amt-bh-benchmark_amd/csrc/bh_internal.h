@@ -161,9 +161,10 @@ hipError_t launch_tree_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_
 // deeper than depth_plan it sets kAbortDepth.
 hipError_t launch_tree_emit(const uint8_t* L, const uint8_t* H, uint32_t n, const uint32_t* tile_prefix, TreeDev t,
                             uint32_t depth_plan, hipStream_t s);
-// Links, geometry and the upward pass of levels depth_plan..0 (one launch
-// per level, each grid-striding over the level's range read on the device;
-// n_sys systems bound a level's node count by n_sys * 8^d).
+// Links, geometry and the upward pass: every leaf in one launch, then the
+// interior nodes of levels depth_plan - 1..0 (one launch per level, each
+// grid-striding over the level's range read on the device; n_sys systems
+// bound a level's node count by n_sys * 8^d).
 hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_t n, const uint16_t* sid, uint32_t n_sys,
                               const double* lohi, uint32_t depth_plan, TreeDev t, hipStream_t s);
 // Marks (node + 1 at its first particle) and lists the topmost nodes with

@@ -302,118 +302,188 @@ __global__ __launch_bounds__(kBlock) void k_tree_links(const uint64_t* __restric
   }
 }
 
-// finalize_node for every node of one level (flat_tree.h:257-274), children
-// already final. Also derives the walk's node records.
-__global__ __launch_bounds__(kBlock) void k_upward(const double4* __restrict__ posm, int level, TreeDev t) {
+// finalize_raw_moment (multipole.h:76-87) of node i from its raw sums about
+// the cell centre, and the node's outputs: FP64 box, mass, com, second
+// moments, the FP64 quadrupole and the walk's node record.
+__device__ inline void finalize_write(TreeDev t, uint32_t i, const double4& cc, double rm, const double* m1,
+                                      const double* m2, const double* blo, const double* bhi, uint32_t cnt,
+                                      bool leaf) {
+  const double inv = 1.0 / rm;
+  const double dl[3] = {m1[0] * inv, m1[1] * inv, m1[2] * inv};
+  const double com[3] = {cc.x + dl[0], cc.y + dl[1], cc.z + dl[2]};
+  double ib[6];
+  ib[0] = m2[0] - rm * dl[0] * dl[0];
+  ib[1] = m2[1] - rm * dl[1] * dl[1];
+  ib[2] = m2[2] - rm * dl[2] * dl[2];
+  ib[3] = m2[3] - rm * dl[0] * dl[1];
+  ib[4] = m2[4] - rm * dl[0] * dl[2];
+  ib[5] = m2[5] - rm * dl[1] * dl[2];
+  for (int a = 0; a < 3; ++a) {
+    t.nbox[6 * uint64_t(i) + a] = blo[a];
+    t.nbox[6 * uint64_t(i) + 3 + a] = bhi[a];
+  }
+  t.mass[i] = rm;
+  for (int a = 0; a < 3; ++a) t.com[3 * i + a] = com[a];
+  for (int a = 0; a < 6; ++a) t.iab[6 * i + a] = ib[a];
+  // walk records: MAC data, links, traceless quadrupole (multipole.h:185-188)
+  const double edge = 2.0 * cc.w;
+  const double tr = ib[0] + ib[1] + ib[2];
+  const double q[6] = {3.0 * ib[0] - tr, 3.0 * ib[1] - tr, 3.0 * ib[2] - tr, 3.0 * ib[3], 3.0 * ib[4], 3.0 * ib[5]};
+  double* q64 = &t.q64[8 * i];
+  q64[0] = rm;
+  for (int a = 0; a < 6; ++a) q64[1 + a] = q[a];
+  q64[7] = 0.0;
+  NodeRec r;
+  r.cx = com[0];
+  r.cy = com[1];
+  r.cz = com[2];
+  r.edge2 = edge * edge;
+  r.first = leaf ? t.begin[i] : t.first_child[i];
+  r.count = leaf ? (cnt | 0x80000000u) : static_cast<uint32_t>(t.nchild[i]);
+  r.m = static_cast<float>(rm);
+  r.qxx = static_cast<float>(q[0]);
+  r.qyy = static_cast<float>(q[1]);
+  r.qxy = static_cast<float>(q[3]);
+  r.qxz = static_cast<float>(q[4]);
+  r.qyz = static_cast<float>(q[5]);
+  t.rec[i] = r;
+}
+
+// finalize_node (flat_tree.h:257-274) of every leaf, at any depth, in one
+// launch. A block takes 256 consecutive leaves of the leaf list (particle
+// order): their particles are one contiguous range, which the block loads
+// into LDS in coalesced chunks of kUpChunk; each thread then adds the
+// particles of its leaf that lie in the chunk, in order, so its sums run
+// over the leaf's particles in the reference's order.
+constexpr uint32_t kUpChunk = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_upward_leaves(const double4* __restrict__ posm, uint32_t n, TreeDev t) {
+  // one array per coordinate: lanes read scattered particles, and 8-byte
+  // elements spread those reads over the banks better than 32-byte ones
+  __shared__ double s_x[kUpChunk], s_y[kUpChunk], s_z[kUpChunk], s_m[kUpChunk];
   if (t.misc[kMiscAbort]) return;
-  const uint32_t lo = t.level_off[level], hi = t.level_off[level + 1];
-  for (uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x; i < hi; i += gridDim.x * kBlock) {
-    const double4 cc = t.ctr[i];
-    const double ref[3] = {cc.x, cc.y, cc.z};
+  const uint32_t n_leaves = t.misc[kMiscLevels + kLevels];
+  const double inf = __builtin_huge_val();
+  for (uint32_t l0 = blockIdx.x * kBlock; l0 < n_leaves; l0 += gridDim.x * kBlock) {
+    const uint32_t lr = l0 + threadIdx.x;
+    const bool mine = lr < n_leaves;
+    const uint32_t l1 = l0 + kBlock < n_leaves ? l0 + kBlock : n_leaves;
+    const uint32_t span_lo = t.leaf_first[l0];
+    const uint32_t span_hi = l1 < n_leaves ? t.leaf_first[l1] : n;
+    uint32_t i = 0, b = 0, e = 0;
+    double4 cc = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (mine) {
+      i = t.leaf_node[lr];
+      b = t.leaf_first[lr];
+      e = b + t.count[i];
+      cc = t.ctr[i];
+    }
     double rm = 0.0, m1[3] = {0.0, 0.0, 0.0}, m2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // the FP64 box of the node's particles (multipole.h:115-128; exact, in
-    // any order): a leaf's from its particles, a parent's from its children
-    const double inf = __builtin_huge_val();
+    // the FP64 box of the node's particles (multipole.h:115-128; exact)
     double blo[3] = {inf, inf, inf}, bhi[3] = {-inf, -inf, -inf};
-    uint32_t cnt;
-    const bool leaf = t.leaf[i] != 0;
-    if (leaf) {
-      const uint32_t b = t.begin[i];
-      cnt = t.count[i];
-      // the particles' loads go out eight at a time (the sums still run in
-      // particle order)
-      constexpr uint32_t kAhead = 8;
-      for (uint32_t p0 = b; p0 < b + cnt; p0 += kAhead) {
-        const uint32_t m = b + cnt - p0 < kAhead ? b + cnt - p0 : kAhead;
-        double4 qs[kAhead];
-#pragma unroll
-        for (uint32_t u = 0; u < kAhead; ++u) qs[u] = posm[p0 + (u < m ? u : 0)];
-#pragma unroll
-        for (uint32_t u = 0; u < kAhead; ++u) {
-          if (u >= m) continue;
-          const double4 q = qs[u];
-          const double mass = q.w;
-          blo[0] = fmin(blo[0], q.x);
-          blo[1] = fmin(blo[1], q.y);
-          blo[2] = fmin(blo[2], q.z);
-          bhi[0] = fmax(bhi[0], q.x);
-          bhi[1] = fmax(bhi[1], q.y);
-          bhi[2] = fmax(bhi[2], q.z);
-          const double d[3] = {q.x - ref[0], q.y - ref[1], q.z - ref[2]};
-          rm += mass;
-          for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
-          m2[0] += mass * d[0] * d[0];
-          m2[1] += mass * d[1] * d[1];
-          m2[2] += mass * d[2] * d[2];
-          m2[3] += mass * d[0] * d[1];
-          m2[4] += mass * d[0] * d[2];
-          m2[5] += mass * d[1] * d[2];
-        }
+    for (uint32_t c0 = span_lo; c0 < span_hi; c0 += kUpChunk) {
+      const uint32_t cn = span_hi - c0 < kUpChunk ? span_hi - c0 : kUpChunk;
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < cn; k += kBlock) {
+        const double4 q = posm[c0 + k];
+        s_x[k] = q.x;
+        s_y[k] = q.y;
+        s_z[k] = q.z;
+        s_m[k] = q.w;
       }
-    } else {
-      const uint32_t fc = t.first_child[i], k = t.nchild[i];
-      cnt = 0;
-      for (uint32_t c = fc; c < fc + k; ++c) {
-        const double mass = t.mass[c];
-        const double d[3] = {t.com[3 * c + 0] - ref[0], t.com[3 * c + 1] - ref[1], t.com[3 * c + 2] - ref[2]};
+      __syncthreads();
+      const uint32_t lo = b > c0 ? b : c0;
+      const uint32_t hi = e < c0 + cn ? e : c0 + cn;
+      for (uint32_t pidx = lo; pidx < hi; ++pidx) {
+        const uint32_t o = pidx - c0;
+        const double x = s_x[o], y = s_y[o], z = s_z[o], mass = s_m[o];
+        blo[0] = fmin(blo[0], x);
+        blo[1] = fmin(blo[1], y);
+        blo[2] = fmin(blo[2], z);
+        bhi[0] = fmax(bhi[0], x);
+        bhi[1] = fmax(bhi[1], y);
+        bhi[2] = fmax(bhi[2], z);
+        const double d[3] = {x - cc.x, y - cc.y, z - cc.z};
         rm += mass;
         for (int a = 0; a < 3; ++a) m1[a] += d[a] * mass;
-        const double* ib = &t.iab[6 * c];
-        m2[0] += ib[0] + mass * d[0] * d[0];
-        m2[1] += ib[1] + mass * d[1] * d[1];
-        m2[2] += ib[2] + mass * d[2] * d[2];
-        m2[3] += ib[3] + mass * d[0] * d[1];
-        m2[4] += ib[4] + mass * d[0] * d[2];
-        m2[5] += ib[5] + mass * d[1] * d[2];
-        cnt += t.count[c];
-        const double* cb = &t.nbox[6 * uint64_t(c)];
-        for (int a = 0; a < 3; ++a) {
-          blo[a] = fmin(blo[a], cb[a]);
-          bhi[a] = fmax(bhi[a], cb[3 + a]);
-        }
+        m2[0] += mass * d[0] * d[0];
+        m2[1] += mass * d[1] * d[1];
+        m2[2] += mass * d[2] * d[2];
+        m2[3] += mass * d[0] * d[1];
+        m2[4] += mass * d[0] * d[2];
+        m2[5] += mass * d[1] * d[2];
       }
+    }
+    if (mine) finalize_write(t, i, cc, rm, m1, m2, blo, bhi, e - b, true);
+  }
+}
+
+// finalize_node for the interior nodes of one level, children already
+// final. A team of 8 lanes takes a node: lane j loads child j (at most 8)
+// and forms its terms as the reference does (mass, m d, the child's second
+// moment + m d_a d_b, its box); the team then adds the terms in child
+// order through shuffles, so the sums are the reference's to the bit, and
+// every lane holds them; lane 0 writes the node. (Loading the children one
+// after another from one thread left each level waiting on ~8 dependent
+// memory round trips.)
+constexpr int kKids = 8;
+
+__global__ __launch_bounds__(kBlock) void k_upward(int level, TreeDev t) {
+  if (t.misc[kMiscAbort]) return;
+  const uint32_t lo = t.level_off[level], hi = t.level_off[level + 1];
+  const double inf = __builtin_huge_val();
+  const int j = static_cast<int>(threadIdx.x) & (kKids - 1);
+  const uint32_t teams = gridDim.x * (kBlock / kKids);
+  for (uint32_t i0 = lo + blockIdx.x * (kBlock / kKids); i0 < hi; i0 += teams) {
+    const uint32_t i = i0 + threadIdx.x / kKids;
+    const bool node_ok = i < hi && !t.leaf[i < hi ? i : lo];
+    const uint32_t fc = node_ok ? t.first_child[i] : 0u;
+    const uint32_t k = node_ok ? t.nchild[i] : 0u;
+    const double4 cc = node_ok ? t.ctr[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    // this lane's child and its terms
+    const bool has = static_cast<uint32_t>(j) < k;
+    const uint32_t c = fc + static_cast<uint32_t>(j);
+    double tm = 0.0, t1[3] = {0.0, 0.0, 0.0}, t2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double clo[3] = {inf, inf, inf}, chi[3] = {-inf, -inf, -inf};
+    uint32_t ccount = 0;
+    if (has) {
+      const double mass = t.mass[c];
+      const double d[3] = {t.com[3 * c + 0] - cc.x, t.com[3 * c + 1] - cc.y, t.com[3 * c + 2] - cc.z};
+      tm = mass;
+      for (int a = 0; a < 3; ++a) t1[a] = d[a] * mass;
+      const double* ib = &t.iab[6 * c];
+      t2[0] = ib[0] + mass * d[0] * d[0];
+      t2[1] = ib[1] + mass * d[1] * d[1];
+      t2[2] = ib[2] + mass * d[2] * d[2];
+      t2[3] = ib[3] + mass * d[0] * d[1];
+      t2[4] = ib[4] + mass * d[0] * d[2];
+      t2[5] = ib[5] + mass * d[1] * d[2];
+      ccount = t.count[c];
+      const double* cb = &t.nbox[6 * uint64_t(c)];
+      for (int a = 0; a < 3; ++a) {
+        clo[a] = cb[a];
+        chi[a] = cb[3 + a];
+      }
+    }
+    // sums in child order; the team's lanes share k, so every shuffle reads
+    // a lane of the team that is in the loop
+    double rm = 0.0, m1[3] = {0.0, 0.0, 0.0}, m2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double blo[3] = {inf, inf, inf}, bhi[3] = {-inf, -inf, -inf};
+    uint32_t cnt = 0;
+    for (uint32_t q = 0; q < k; ++q) {
+      rm += __shfl(tm, static_cast<int>(q), kKids);
+      for (int a = 0; a < 3; ++a) m1[a] += __shfl(t1[a], static_cast<int>(q), kKids);
+      for (int a = 0; a < 6; ++a) m2[a] += __shfl(t2[a], static_cast<int>(q), kKids);
+      for (int a = 0; a < 3; ++a) {
+        blo[a] = fmin(blo[a], __shfl(clo[a], static_cast<int>(q), kKids));
+        bhi[a] = fmax(bhi[a], __shfl(chi[a], static_cast<int>(q), kKids));
+      }
+      cnt += static_cast<uint32_t>(__shfl(static_cast<int>(ccount), static_cast<int>(q), kKids));
+    }
+    if (node_ok && j == 0) {
       t.count[i] = cnt;
+      finalize_write(t, i, cc, rm, m1, m2, blo, bhi, cnt, false);
     }
-    // finalize_raw_moment (multipole.h:76-87)
-    const double inv = 1.0 / rm;
-    const double dl[3] = {m1[0] * inv, m1[1] * inv, m1[2] * inv};
-    const double com[3] = {ref[0] + dl[0], ref[1] + dl[1], ref[2] + dl[2]};
-    double ib[6];
-    ib[0] = m2[0] - rm * dl[0] * dl[0];
-    ib[1] = m2[1] - rm * dl[1] * dl[1];
-    ib[2] = m2[2] - rm * dl[2] * dl[2];
-    ib[3] = m2[3] - rm * dl[0] * dl[1];
-    ib[4] = m2[4] - rm * dl[0] * dl[2];
-    ib[5] = m2[5] - rm * dl[1] * dl[2];
-    for (int a = 0; a < 3; ++a) {
-      t.nbox[6 * uint64_t(i) + a] = blo[a];
-      t.nbox[6 * uint64_t(i) + 3 + a] = bhi[a];
-    }
-    t.mass[i] = rm;
-    for (int a = 0; a < 3; ++a) t.com[3 * i + a] = com[a];
-    for (int a = 0; a < 6; ++a) t.iab[6 * i + a] = ib[a];
-    // walk records: MAC data, links, traceless quadrupole (multipole.h:185-188)
-    const double edge = 2.0 * cc.w;
-    const double tr = ib[0] + ib[1] + ib[2];
-    const double q[6] = {3.0 * ib[0] - tr, 3.0 * ib[1] - tr, 3.0 * ib[2] - tr, 3.0 * ib[3], 3.0 * ib[4], 3.0 * ib[5]};
-    double* q64 = &t.q64[8 * i];
-    q64[0] = rm;
-    for (int a = 0; a < 6; ++a) q64[1 + a] = q[a];
-    q64[7] = 0.0;
-    NodeRec r;
-    r.cx = com[0];
-    r.cy = com[1];
-    r.cz = com[2];
-    r.edge2 = edge * edge;
-    r.first = leaf ? t.begin[i] : t.first_child[i];
-    r.count = leaf ? (cnt | 0x80000000u) : static_cast<uint32_t>(t.nchild[i]);
-    r.m = static_cast<float>(rm);
-    r.qxx = static_cast<float>(q[0]);
-    r.qyy = static_cast<float>(q[1]);
-    r.qxy = static_cast<float>(q[3]);
-    r.qxz = static_cast<float>(q[4]);
-    r.qyz = static_cast<float>(q[5]);
-    t.rec[i] = r;
   }
 }
 
@@ -557,15 +627,17 @@ hipError_t launch_tree_finish(const uint64_t* keys, const double4* posm, uint32_
                               const double* lohi, uint32_t depth_plan, TreeDev t, hipStream_t s) {
   // grid-stride loops over the node count read on the device
   hipLaunchKernelGGL(k_tree_links, dim3(grid_for(t.cap, 4096)), dim3(kBlock), 0, s, keys, n, sid, lohi, t);
-  // bottom-up, one launch per level up to the planned depth; each reads its
-  // level's range on the device and grid-strides over it. Level d holds at
-  // most min(n_sys * 8^d, n) nodes, which sizes its grid.
+  // every leaf, then the interior nodes bottom-up: one launch per level
+  // below the planned depth (the deepest level holds leaves only), each
+  // reading its level's range on the device and grid-striding over it.
+  // Level d holds at most min(n_sys * 8^d, n) nodes, which sizes its grid.
+  hipLaunchKernelGGL(k_upward_leaves, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, posm, n, t);
   const int top = static_cast<int>(depth_plan < kDepthCap ? depth_plan : kDepthCap);
-  for (int d = top; d >= 0; --d) {
+  for (int d = top - 1; d >= 0; --d) {
     uint64_t bound = n_sys;
     for (int e = 0; e < d && bound < n; ++e) bound *= 8;
     if (bound > n) bound = n;
-    hipLaunchKernelGGL(k_upward, dim3(grid_for(bound, 2048)), dim3(kBlock), 0, s, posm, d, t);
+    hipLaunchKernelGGL(k_upward, dim3(grid_for(bound * kKids, 4096)), dim3(kBlock), 0, s, d, t);
   }
   return hipGetLastError();
 }
