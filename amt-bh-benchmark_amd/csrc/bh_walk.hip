@@ -316,16 +316,19 @@ __device__ inline void eval_pc_group(const PcRing<Real>& r, int j0, int C, const
 
 // Evaluator of a target's particles: lane = (particle p, slice s), batches
 // of B = S * C entries, slice s takes entries [s*C, (s+1)*C). Batches are
-// summed in plain FP32 and folded into the compensated accumulator every
-// kFold batches (small targets have short per-lane batches).
+// summed in plain FP32 and folded into the compensated accumulator once a
+// lane has summed kFoldEntries entries since the last fold (two full
+// single-slice batches; small targets have many slices and short per-lane
+// batches, so they fold after more batches).
 template <typename Real, bool kMask>
 struct TargetEval {
   using V4 = typename Vec4T<Real>::T;
-  static constexpr int kFold = 2;
+  static constexpr int kFoldEntries = 2 * kBatch;
   Real tx, ty, tz, eps2;
   int S, C, s;  // slices, entries per slice and batch, this lane's slice
   bool active;
   uint32_t B;
+  int fold_every = 1;  // batches between folds: max(1, kFoldEntries / C)
   int pending = 0;
   Real px = 0, py = 0, pz = 0;
   Acc<Real> acc;
@@ -340,11 +343,11 @@ struct TargetEval {
   __device__ inline void restore() { C = static_cast<int>(B) / S; }
   __device__ inline void pp(const V4* ring) {
     if (active) eval_pp_part<Real, kMask>(&ring[s * C], C, tx, ty, tz, eps2, px, py, pz);
-    if (++pending == kFold) fold();
+    if (++pending >= fold_every) fold();
   }
   __device__ inline void pc(const PcRing<Real>& ring) {
     if (active) eval_pc_part<Real>(ring, s * C, C, tx, ty, tz, eps2, px, py, pz);
-    if (++pending == kFold) fold();
+    if (++pending >= fold_every) fold();
   }
 };
 
@@ -889,6 +892,7 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
     ev.S = S;
     ev.C = kBatch / S;
     ev.B = static_cast<uint32_t>(S * ev.C);
+    ev.fold_every = ev.C >= TargetEval<Real, kMask>::kFoldEntries ? 1 : TargetEval<Real, kMask>::kFoldEntries / ev.C;
     ev.active = lane < static_cast<int>(tc) * S;
     const int p = lane % static_cast<int>(tc);
     ev.s = lane / static_cast<int>(tc);
