@@ -601,9 +601,10 @@ struct Sorted {
 
 // build_flat_tree (flat_build_ttg.h:276-281) of every system of the plan,
 // d_in[0..n) into the scratch; the Morton-ordered particles (each system in
-// its own range) go to d_out. `events`: record the phase events.
+// its own range) go to d_out, here or (walk_writes_out) by the step's walk,
+// which reads them from d_in. `events`: record the phase events.
 int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_in, bh_particle* d_out, bool events,
-                  Sorted* out) {
+                  Sorted* out, bool walk_writes_out) {
   hipStream_t s = c->stream;
   const uint32_t n = p.n;
   const uint32_t tiles = (n + kTreeTile - 1) / kTreeTile;
@@ -623,7 +624,7 @@ int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_
   HIP_OK(c, launch_keys_hist(d_in, n, b.sid, b.lohi, b.keys_a, b.hist, s));
   HIP_OK(c, launch_radix_sort(b.keys_a, b.vals_a, b.keys_b, b.vals_b, n, p.sid_passes, b.sid, b.hist, b.dbase,
                               b.lookback, b.lookback + (lb_words - 10), &out->keys, &out->perm, s));
-  HIP_OK(c, launch_gather(d_in, d_out, out->perm, b.posm, n, s));
+  HIP_OK(c, launch_gather(d_in, walk_writes_out ? nullptr : d_out, out->perm, b.posm, n, s));
   if (events) HIP_OK(c, hipEventRecord(c->ev[1], s));
   HIP_OK(c, launch_tree_count(out->keys, n, b.sid, sys, p.mlx, b.L, b.H, b.tcounts, s));
   HIP_OK(c, launch_tree_scan(b.tcounts, tiles, b.tprefix, b.misc + kMiscLevels, s));
@@ -636,7 +637,7 @@ int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_
 // run_grouped_pass (grouped_ttg.h:297-323) over every system of the tree in
 // the scratch, with each system's parameters from the system table.
 int enqueue_walk(Ctx* c, const Plan& p, const Scratch& b, bh_particle* parts, double* d_acc, uint32_t* d_npp,
-                 uint32_t* d_npc, bool events) {
+                 uint32_t* d_npc, bool events, const bh_particle* src = nullptr, const uint32_t* perm = nullptr) {
   hipStream_t s = c->stream;
   const uint32_t n = p.n;
   const SysDev* sys = reinterpret_cast<const SysDev*>(b.tables);
@@ -672,6 +673,8 @@ int enqueue_walk(Ctx* c, const Plan& p, const Scratch& b, bh_particle* parts, do
   a.n_parts = n;
   a.stack_limit = c->stack_limit;
   a.parts = parts;
+  a.src = src;
+  a.perm = perm;
   a.acc = d_acc;
   a.n_pp = d_npp;
   a.n_pc = d_npc;
@@ -728,8 +731,8 @@ int launch_graph(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_i
   if (!hit) {
     HIP_OK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     Sorted sd;
-    int rc = enqueue_build(c, p, b, d_in, d_out, false, &sd);
-    if (rc == BH_OK) rc = enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, false);
+    int rc = enqueue_build(c, p, b, d_in, d_out, false, &sd, true);
+    if (rc == BH_OK) rc = enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, false, d_in, sd.perm);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
     if (rc != BH_OK) {
@@ -809,8 +812,8 @@ int run(Ctx* c, Plan p, const bh_particle* d_in, bh_particle* d_out, bool walk, 
       TRY(launch_graph(c, p, b, d_in, d_out, &sd));
     } else {
       HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
-      TRY(enqueue_build(c, p, b, d_in, d_out, true, &sd));
-      if (walk) TRY(enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, true));
+      TRY(enqueue_build(c, p, b, d_in, d_out, true, &sd, walk));
+      if (walk) TRY(enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, true, d_in, sd.perm));
     }
     TRY(finish(c, b));
     const uint32_t* m = c->host_misc;
@@ -1189,8 +1192,8 @@ int bh_step_device_noalloc(bh_ctx* h, const bh_particle* d_in, bh_particle* d_ou
   c->reruns = 0;
   Sorted sd;
   HIP_OK(c, hipEventRecord(c->ev[0], c->stream));
-  TRY(enqueue_build(c, p, b, d_in, d_out, true, &sd));
-  TRY(enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, true));
+  TRY(enqueue_build(c, p, b, d_in, d_out, true, &sd, true));
+  TRY(enqueue_walk(c, p, b, d_out, nullptr, nullptr, nullptr, true, d_in, sd.perm));
   TRY(finish(c, b));
   const uint32_t* m = c->host_misc;
   if (m[kMiscNonfinite]) {

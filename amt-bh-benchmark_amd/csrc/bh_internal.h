@@ -125,6 +125,8 @@ struct WalkArgs {
   uint32_t n_parts;        // index never leaves its array, whatever the tree holds
   uint32_t stack_limit;    // frames the walk may use (kLevels; lower only to test the incomplete-walk report)
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
+  const bh_particle* src;  // with parts: the unsorted input, read at perm (else parts is read in place)
+  const uint32_t* perm;    // [n] Morton position -> input index (with src)
   double* acc;             // [3n] G * acceleration (nullable)
   uint32_t* n_pp;          // [n] (nullable)
   uint32_t* n_pc;          // [n] (nullable)
@@ -147,6 +149,8 @@ hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_
                              int sid_passes, const uint16_t* sid, uint32_t* hist, uint32_t* digit_base,
                              uint32_t* lookback, uint32_t* tile_counters, uint64_t** keys_out, uint32_t** vals_out,
                              hipStream_t s);
+// The FP64 walk copy (x, y, z, m) in Morton order; with out, also the
+// Morton-ordered particles (a step's walk writes those itself).
 hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
                          hipStream_t s);
 uint64_t radix_lookback_words(uint32_t n);  // 10 passes of look-back words + 10 tile counters

@@ -333,6 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_onesweep(const uint64_t* __restrict_
 
 // Permutes the particles into Morton order and writes the FP64 walk copy
 // (x, y, z, m), as sort_particles' final loop does (flat_build_ttg.h:180-188).
+// Without out only the walk copy is written (a step's walk writes the
+// Morton-ordered particles from the input itself).
 __global__ __launch_bounds__(kBlock) void k_gather(const bh_particle* __restrict__ in, bh_particle* __restrict__ out,
                                                    const uint32_t* __restrict__ perm, double4* __restrict__ posm, uint32_t n) {
   // four lanes per particle, one 16-byte quarter each: consecutive lanes
@@ -341,8 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const bh_particle* __restrict
   const uint32_t k = t >> 2, part = t & 3u;
   if (k >= n) return;  // whole quads leave together
   const uint32_t j = perm[k];
-  const uint4 v = reinterpret_cast<const uint4*>(in + j)[part];
-  reinterpret_cast<uint4*>(out + k)[part] = v;
+  // the walk copy needs quarters 0-2 only
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (out || part < 3) v = reinterpret_cast<const uint4*>(in + j)[part];
+  if (out) reinterpret_cast<uint4*>(out + k)[part] = v;
   // quarters: 0 = {id, pad, mass}, 1 = {x, y}, 2 = {z, vx}, 3 = {vy, vz}
   const int lane = __lane_id();
   const uint32_t mlo = static_cast<uint32_t>(__shfl(static_cast<int>(v.z), lane ^ 2));

@@ -845,8 +845,10 @@ __device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax
   if (a.n_pp) a.n_pp[k3] = n_pp;
   if (a.n_pc) a.n_pc[k3] = n_pc;
   if (a.parts) {
-    // kick_one / drift_one (integrate.h:15-16): v += a*dt; x += v*dt
-    bh_particle& pt = a.parts[k3];
+    // kick_one / drift_one (integrate.h:15-16): v += a*dt; x += v*dt. The
+    // particle comes from the unsorted input (src at perm) when given: the
+    // whole Morton-ordered record is written here, not by the sort's gather.
+    bh_particle pt = a.src ? a.src[a.perm[k3]] : a.parts[k3];
     const double vx = pt.vel[0] + gx * kick_dt;
     const double vy = pt.vel[1] + gy * kick_dt;
     const double vz = pt.vel[2] + gz * kick_dt;
@@ -858,6 +860,7 @@ __device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax
       pt.pos[1] = pt.pos[1] + vy * drift_dt;
       pt.pos[2] = pt.pos[2] + vz * drift_dt;
     }
+    a.parts[k3] = pt;
   }
 }
 
