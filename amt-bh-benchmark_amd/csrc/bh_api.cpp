@@ -102,6 +102,8 @@ struct Plan {
   bool deep = true;        // walk stack frames beyond LDS
   uint32_t slots = 0;      // walk workgroups
   bool walk = true;        // false: tree build only
+  bool carry_out = false;  // the walk writes the compact positions of its output (resident steps)
+  bool carried_in = false; // the input is the last resident step's output: build from its compact positions
 };
 
 // One captured step, keyed by everything its launches depend on.
@@ -111,7 +113,7 @@ struct GraphKey {
   uint64_t gen;
   uint32_t n, S, n_chunks, mlx, cap_nodes, depth_plan, slots;
   uint32_t table_bytes;
-  uint8_t deep, unsoftened, precision, mode;
+  uint8_t deep, unsoftened, precision, mode, carry_out, carried_in;
   bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof *this) == 0; }
 };
 struct GraphEntry {
@@ -140,6 +142,13 @@ struct Ctx {
   DevBuf<bh_particle> pa, pb;    // particle ping-pong of the resident state
   int cur = 0;                   // which of pa/pb holds the resident state
   uint64_t state_n = 0;
+  // {x, y, z, m} of the last resident step's output and its non-finite flag,
+  // valid (carried_ok) until anything else runs or the state is exposed
+  DevBuf<double4> carry;
+  DevBuf<uint32_t> carry_flag;
+  bool carried_ok = false;
+  const void* carried_out = nullptr;
+  uint64_t carried_n = 0;
   DevBuf<bh_particle> tmp_in, tmp_out;  // host-path staging (bh_step, bh_step_batch_host, bh_build_tree)
   DevBuf<uint64_t> keys_a, keys_b;
   DevBuf<uint32_t> vals_a, vals_b;
@@ -453,6 +462,10 @@ int provision(Ctx* c, const Plan& p, Scratch* b) {
   TRY(ensure(c, c->nbox, 6ull * m, "nodes"));
   TRY(ensure(c, c->leaf_node, m, "leaves"));
   TRY(ensure(c, c->leaf_first, m, "leaves"));
+  if (p.carry_out || p.carried_in) {
+    TRY(ensure(c, c->carry, n, "carried positions"));
+    TRY(ensure(c, c->carry_flag, 1, "carried positions"));
+  }
   if (p.walk) {
     TRY(ensure(c, c->mark, n, "targets"));
     TRY(ensure(c, c->targets, n, "targets"));
@@ -620,8 +633,14 @@ int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_
     HIP_OK(c, hipMemcpyAsync(b.hist + 8 * 256, c->host_tables + p.off_hist, 2 * 256 * 4, hipMemcpyHostToDevice, s));
   HIP_OK(c, launch_fill_u32(b.lookback, 0u, lb_words, s));
   HIP_OK(c, launch_fill_sid(chunks, p.n_chunks, b.sid, s));
-  HIP_OK(c, launch_bounds(d_in, chunks, p.n_chunks, chunk_first, p.S, b.partial, b.misc + kMiscNonfinite, b.lohi, s));
-  HIP_OK(c, launch_keys_hist(d_in, n, b.sid, b.lohi, b.keys_a, b.hist, s));
+  if (p.carried_in) {
+    HIP_OK(c, launch_bounds(c->carry.p, c->carry_flag.p, chunks, p.n_chunks, chunk_first, p.S, b.partial,
+                            b.misc + kMiscNonfinite, b.lohi, s));
+    HIP_OK(c, launch_keys_hist(c->carry.p, n, b.sid, b.lohi, b.keys_a, b.hist, s));
+  } else {
+    HIP_OK(c, launch_bounds(d_in, chunks, p.n_chunks, chunk_first, p.S, b.partial, b.misc + kMiscNonfinite, b.lohi, s));
+    HIP_OK(c, launch_keys_hist(d_in, n, b.sid, b.lohi, b.keys_a, b.hist, s));
+  }
   HIP_OK(c, launch_radix_sort(b.keys_a, b.vals_a, b.keys_b, b.vals_b, n, p.sid_passes, b.sid, b.hist, b.dbase,
                               b.lookback, b.lookback + (lb_words - 10), &out->keys, &out->perm, s));
   HIP_OK(c, launch_gather(d_in, walk_writes_out ? nullptr : d_out, out->perm, b.posm, n, s));
@@ -675,6 +694,14 @@ int enqueue_walk(Ctx* c, const Plan& p, const Scratch& b, bh_particle* parts, do
   a.parts = parts;
   a.src = src;
   a.perm = perm;
+  a.carry = nullptr;
+  a.carry_bad = nullptr;
+  if (parts && p.carry_out) {
+    // read by this step's build already (carried_in): cleared for the walk's
+    HIP_OK(c, launch_fill_u32(c->carry_flag.p, 0u, 1, s));
+    a.carry = c->carry.p;
+    a.carry_bad = c->carry_flag.p;
+  }
   a.acc = d_acc;
   a.n_pp = d_npp;
   a.n_pc = d_npc;
@@ -713,6 +740,8 @@ GraphKey graph_key(const Ctx* c, const Plan& p, const void* in, const void* out)
   k.unsoftened = p.unsoftened;
   k.precision = static_cast<uint8_t>(c->precision);
   k.mode = static_cast<uint8_t>(c->walk_mode);
+  k.carry_out = p.carry_out ? 1 : 0;
+  k.carried_in = p.carried_in ? 1 : 0;
   return k;
 }
 
@@ -801,6 +830,7 @@ void take_report(Ctx* c, const Plan& p, const Scratch& b, const Sorted& sd, bh_p
 int run(Ctx* c, Plan p, const bh_particle* d_in, bh_particle* d_out, bool walk, bh_step_timing* timing) {
   c->have_stats = false;
   c->has_tree = false;
+  c->carried_ok = false;
   for (int attempt = 0;; ++attempt) {
     c->reruns = static_cast<uint32_t>(attempt);
     plan_shape(c, &p, walk);
@@ -861,30 +891,41 @@ int run(Ctx* c, Plan p, const bh_particle* d_in, bh_particle* d_out, bool walk, 
       set_err(c, "incomplete tree: the walk met an interior node it could not open");
       return BH_ERR_INCOMPLETE;
     }
+    if (walk && p.carry_out) {
+      c->carried_ok = true;
+      c->carried_out = d_out;
+      c->carried_n = p.n;
+    }
     return BH_OK;
   }
 }
 
 // One step of a batch: systems sys[0..S) own consecutive ranges of d_in.
-int step_batch(Ctx* c, const bh_particle* d_in, bh_particle* d_out, const HostSys* sys, uint32_t S, bh_step_timing* timing) {
+// resident: a step of the resident state; its walk keeps the output's
+// compact positions, which the next resident step builds from when its input
+// is that output and nothing else ran in between.
+int step_batch(Ctx* c, const bh_particle* d_in, bh_particle* d_out, const HostSys* sys, uint32_t S, bh_step_timing* timing,
+               bool resident = false) {
   if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
   HIP_OK(c, hipSetDevice(c->device));
   Plan p;
   TRY(plan_systems(c, sys, S, false, &p));
   if (p.n == 0) return BH_OK;
+  p.carry_out = resident;
+  p.carried_in = resident && c->carried_ok && c->carried_out == d_in && c->carried_n == p.n;
   return run(c, p, d_in, d_out, true, timing);
 }
 
 int step_device(Ctx* c, const bh_particle* d_in, bh_particle* d_out, uint64_t n, const bh_force_params* fp,
                 uint32_t max_leaf, uint32_t group_size, uint32_t target_size, double kick_dt, double drift_dt,
-                bh_step_timing* timing) {
+                bh_step_timing* timing, bool resident = false) {
   TRY(check_args(c, n, fp, max_leaf, group_size));
   if (n == 0) {
     if (timing) *timing = bh_step_timing{0.0, 0.0, 0.0};
     return BH_OK;
   }
   const HostSys one{n, *fp, max_leaf, group_size, target_size, kick_dt, drift_dt, 0};
-  return step_batch(c, d_in, d_out, &one, 1, timing);
+  return step_batch(c, d_in, d_out, &one, 1, timing, resident);
 }
 
 std::vector<HostSys> host_systems(const bh_system* systems, uint32_t S) {
@@ -1009,6 +1050,8 @@ void bh_ctx_destroy(bh_ctx* h) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
   release(c->pa);
+  release(c->carry);
+  release(c->carry_flag);
   release(c->pb);
   release(c->tmp_in);
   release(c->tmp_out);
@@ -1291,6 +1334,7 @@ int bh_state_upload(bh_ctx* h, const bh_particle* host, uint64_t n) {
   HIP_OK(c, hipSetDevice(c->device));
   TRY(ensure(c, c->pa, n, "particles"));
   TRY(ensure(c, c->pb, n, "particles"));
+  c->carried_ok = false;
   if (n) HIP_OK(c, hipMemcpyAsync(c->pa.p, host, n * sizeof(bh_particle), hipMemcpyHostToDevice, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   c->cur = 0;
@@ -1308,7 +1352,7 @@ int bh_state_step(bh_ctx* h, const bh_force_params* fp, uint32_t max_leaf, uint3
   }
   bh_particle* in = c->cur == 0 ? c->pa.p : c->pb.p;
   bh_particle* out = c->cur == 0 ? c->pb.p : c->pa.p;
-  TRY(step_device(c, in, out, c->state_n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing));
+  TRY(step_device(c, in, out, c->state_n, fp, max_leaf, group_size, target_size, kick_dt, drift_dt, timing, true));
   c->cur ^= 1;
   return BH_OK;
 }
@@ -1330,7 +1374,7 @@ int bh_state_step_batch(bh_ctx* h, const bh_system* systems, uint32_t n_systems,
   }
   bh_particle* in = c->cur == 0 ? c->pa.p : c->pb.p;
   bh_particle* out = c->cur == 0 ? c->pb.p : c->pa.p;
-  TRY(step_batch(c, in, out, v.data(), n_systems, timing));
+  TRY(step_batch(c, in, out, v.data(), n_systems, timing, true));
   c->cur ^= 1;
   return BH_OK;
 }
@@ -1338,6 +1382,8 @@ int bh_state_step_batch(bh_ctx* h, const bh_system* systems, uint32_t n_systems,
 bh_particle* bh_state_device_ptr(bh_ctx* h) {
   Ctx* c = as(h);
   if (!c) return nullptr;
+  // the caller may write the state: the next step reads it afresh
+  c->carried_ok = false;
   return c->cur == 0 ? c->pa.p : c->pb.p;
 }
 

@@ -127,6 +127,8 @@ struct WalkArgs {
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
   const bh_particle* src;  // with parts: the unsorted input, read at perm (else parts is read in place)
   const uint32_t* perm;    // [n] Morton position -> input index (with src)
+  double4* carry;          // with parts: [n] the new {x, y, z, m} for the next step's build (nullable)
+  uint32_t* carry_bad;     // with carry: set when a new position, velocity or mass is not finite
   double* acc;             // [3n] G * acceleration (nullable)
   uint32_t* n_pp;          // [n] (nullable)
   uint32_t* n_pc;          // [n] (nullable)
@@ -141,6 +143,15 @@ hipError_t launch_fill_sid(const uint4* chunks, uint32_t n_chunks, uint16_t* sid
 hipError_t launch_bounds(const bh_particle* p, const uint4* chunks, uint32_t n_chunks, const uint32_t* chunk_first,
                          uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s);
 hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
+                            uint32_t* hist, hipStream_t s);
+// The same from the compact positions {x, y, z, m} a resident step's walk
+// wrote for its output (the next step's input): half the bytes of the
+// particle records. carried_bad: that walk's non-finite flag, added to
+// nonfinite.
+hipError_t launch_bounds(const double4* pos, const uint32_t* carried_bad, const uint4* chunks, uint32_t n_chunks,
+                         const uint32_t* chunk_first, uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi,
+                         hipStream_t s);
+hipError_t launch_keys_hist(const double4* pos, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
                             uint32_t* hist, hipStream_t s);
 // 8 passes over the 63-bit keys, then sid_passes (0-2) over the system id
 // taken from sid[index]: a stable sort of each system's (key, index) pairs

@@ -82,20 +82,40 @@ __global__ __launch_bounds__(kBlock) void k_fill_sid(const uint4* __restrict__ c
   for (uint32_t i = c.y + threadIdx.x; i < c.z; i += kBlock) sid[i] = static_cast<uint16_t>(c.x);
 }
 
+// A particle's position for the build, from its record or from the compact
+// {x, y, z, m} copy; `finite` checks what the source holds (the compact
+// copy's velocities were checked by the walk that wrote it).
+__device__ inline void pos_of(const bh_particle& q, double* x, double* y, double* z, bool* finite) {
+  *x = q.pos[0];
+  *y = q.pos[1];
+  *z = q.pos[2];
+  *finite = isfinite(*x) && isfinite(*y) && isfinite(*z) && isfinite(q.mass) && isfinite(q.vel[0]) &&
+            isfinite(q.vel[1]) && isfinite(q.vel[2]);
+}
+__device__ inline void pos_of(const double4& q, double* x, double* y, double* z, bool* finite) {
+  *x = q.x;
+  *y = q.y;
+  *z = q.z;
+  *finite = isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w);
+}
+
 // Min/max of the positions of one chunk (flat_build_ttg.h:93-113; min/max
 // is exact, so the chunking does not matter); flags any non-finite
-// position, velocity or mass.
-__global__ __launch_bounds__(kBlock) void k_bounds_partial(const bh_particle* __restrict__ p, const uint4* __restrict__ chunks,
-                                                           double* __restrict__ partial, uint32_t* __restrict__ nonfinite) {
+// position, velocity or mass (and, from a compact copy, its writer's flag).
+template <typename Src>
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(const Src* __restrict__ p, const uint32_t* __restrict__ carried_bad,
+                                                           const uint4* __restrict__ chunks, double* __restrict__ partial,
+                                                           uint32_t* __restrict__ nonfinite) {
   const double inf = __builtin_huge_val();
   double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
   bool bad = false;
+  if (carried_bad && blockIdx.x == 0 && threadIdx.x == 0 && *carried_bad) atomicOr(nonfinite, 1u);
   const uint4 c = chunks[blockIdx.x];
   for (uint32_t i = c.y + threadIdx.x; i < c.z; i += kBlock) {
-    const bh_particle& q = p[i];
-    const double x = q.pos[0], y = q.pos[1], z = q.pos[2];
-    bad |= !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(q.mass) && isfinite(q.vel[0]) && isfinite(q.vel[1]) &&
-             isfinite(q.vel[2]));
+    double x, y, z;
+    bool fin;
+    pos_of(p[i], &x, &y, &z, &fin);
+    bad |= !fin;
     lo[0] = fmin(lo[0], x);
     lo[1] = fmin(lo[1], y);
     lo[2] = fmin(lo[2], z);
@@ -153,7 +173,8 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const double* __restric
 
 // Morton key per particle (input order) and the 8 digit histograms of all
 // keys, for the onesweep passes' global digit offsets.
-__global__ __launch_bounds__(kBlock) void k_keys_hist(const bh_particle* __restrict__ p, uint32_t n,
+template <typename Src>
+__global__ __launch_bounds__(kBlock) void k_keys_hist(const Src* __restrict__ p, uint32_t n,
                                                       const uint16_t* __restrict__ sid, const double* __restrict__ lohi,
                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[8 * 256];
@@ -164,9 +185,11 @@ __global__ __launch_bounds__(kBlock) void k_keys_hist(const bh_particle* __restr
     domain_from_lohi(lohi + 6 * sid[i], c, &hw);
     const double scale = 2097152.0 / (2.0 * hw);
     const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
-    const bh_particle& q = p[i];
-    const uint64_t key = spread_bits3(quantize(q.pos[0], lx, scale)) | (spread_bits3(quantize(q.pos[1], ly, scale)) << 1) |
-                         (spread_bits3(quantize(q.pos[2], lz, scale)) << 2);
+    double x, y, z;
+    bool fin;
+    pos_of(p[i], &x, &y, &z, &fin);
+    const uint64_t key = spread_bits3(quantize(x, lx, scale)) | (spread_bits3(quantize(y, ly, scale)) << 1) |
+                         (spread_bits3(quantize(z, lz, scale)) << 2);
     keys[i] = key;
 #pragma unroll
     for (int b = 0; b < 8; ++b) atomicAdd(&s_h[b * 256 + ((key >> (8 * b)) & 255u)], 1u);
@@ -378,14 +401,30 @@ hipError_t launch_fill_sid(const uint4* chunks, uint32_t n_chunks, uint16_t* sid
 
 hipError_t launch_bounds(const bh_particle* p, const uint4* chunks, uint32_t n_chunks, const uint32_t* chunk_first,
                          uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s) {
-  hipLaunchKernelGGL(k_bounds_partial, dim3(n_chunks), dim3(kBlock), 0, s, p, chunks, partial, nonfinite);
+  hipLaunchKernelGGL(k_bounds_partial<bh_particle>, dim3(n_chunks), dim3(kBlock), 0, s, p, nullptr, chunks, partial,
+                     nonfinite);
+  hipLaunchKernelGGL(k_bounds_final, dim3(n_sys), dim3(kBlock), 0, s, partial, chunk_first, lohi);
+  return hipGetLastError();
+}
+
+hipError_t launch_bounds(const double4* pos, const uint32_t* carried_bad, const uint4* chunks, uint32_t n_chunks,
+                         const uint32_t* chunk_first, uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds_partial<double4>, dim3(n_chunks), dim3(kBlock), 0, s, pos, carried_bad, chunks, partial,
+                     nonfinite);
   hipLaunchKernelGGL(k_bounds_final, dim3(n_sys), dim3(kBlock), 0, s, partial, chunk_first, lohi);
   return hipGetLastError();
 }
 
 hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
                             uint32_t* hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_keys_hist, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, p, n, sid, lohi, keys, hist);
+  hipLaunchKernelGGL(k_keys_hist<bh_particle>, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, p, n, sid, lohi, keys, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_keys_hist(const double4* pos, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
+                            uint32_t* hist, hipStream_t s) {
+  hipLaunchKernelGGL(k_keys_hist<double4>, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, pos, n, sid, lohi, keys, hist);
   return hipGetLastError();
 }
 

@@ -861,6 +861,13 @@ __device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax
       pt.pos[2] = pt.pos[2] + vz * drift_dt;
     }
     a.parts[k3] = pt;
+    if (a.carry) {
+      // the next step's build reads these instead of the records
+      a.carry[k3] = make_double4(pt.pos[0], pt.pos[1], pt.pos[2], pt.mass);
+      if (!(isfinite(pt.pos[0]) && isfinite(pt.pos[1]) && isfinite(pt.pos[2]) && isfinite(vx) && isfinite(vy) &&
+            isfinite(vz) && isfinite(pt.mass)))
+        atomicOr(a.carry_bad, 1u);
+    }
   }
 }
 

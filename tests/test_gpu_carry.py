@@ -1,0 +1,59 @@
+"""Resident steps build from the compact positions their previous step's
+walk wrote (half the bytes of the particle records); every other path reads
+the records. Bars: a resident leapfrog equals, byte for byte, the same
+leapfrog stepped through bh_step_device on two buffers (which never carries),
+eager and as graph replays; a value that becomes non-finite in one resident
+step is reported by the next, carried or not."""
+import numpy as np
+import pytest
+
+import golden_io
+from bhgpu import devmem, ic, layout
+from bhgpu import lib as bl
+
+pytestmark = pytest.mark.gpu
+
+SCHED = [(2.5e-4, 5e-4)] + [(5e-4, 5e-4)] * 4 + [(2.5e-4, 0.0)]
+
+
+def device_run(ctx, p, fp):
+    n = len(p)
+    buf = [devmem.DeviceBuffer(n * 64) for _ in range(2)]
+    buf[0].upload(p)
+    cur = 0
+    for kick, drift in SCHED:
+        ctx.step_device(buf[cur].ptr, buf[1 - cur].ptr, n, fp, 16, 256, 64, kick, drift)
+        cur = 1 - cur
+    return buf[cur].download(np.empty(n, layout.PARTICLE))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_resident_steps_equal_device_steps(graph):
+    p = ic.plummer(30000, 81)
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    with bl.Context(0) as ctx:
+        ctx.set_graph_mode(bl.BH_GRAPH_ON if graph else bl.BH_GRAPH_OFF)
+        print(f"carried positions graph={graph}", flush=True)
+        want = device_run(ctx, p, fp)
+        ctx.state_upload(p)
+        for kick, drift in SCHED:
+            ctx.state_step(fp, 16, 256, 64, kick, drift)
+        assert golden_io.bytes_equal(ctx.state_download(), want)
+
+
+@pytest.mark.parametrize("expose", [False, True])
+def test_carried_nonfinite_reported(expose):
+    """A drift that overflows a position to infinity: the step that makes it
+    succeeds (its own input is finite), the next reports BH_ERR_NONFINITE,
+    from the carried flag or (state exposed in between) from the records."""
+    p = ic.plummer(5000, 82)
+    p["vel"][123, 0] = 1e300
+    fp = layout.force_params(0.5, 1e-4, 1.0, True)
+    with bl.Context(0) as ctx:
+        ctx.state_upload(p)
+        ctx.state_step(fp, 16, 256, 32, 0.0, 1e10)
+        if expose:
+            ctx.state_device_ptr()
+        with pytest.raises(bl.BHError) as e:
+            ctx.state_step(fp, 16, 256, 32, 0.0, 1e-3)
+        assert e.value.code == 2
