@@ -3,7 +3,7 @@
 Workload (the reference configuration the metric is quoted on): Plummer
 sphere N = 16,777,216 (G = M = a = 1, r < 10a, seed 3), eps = 1e-3,
 theta = 0.5, leaf capacity 32, leapfrog dt = 5e-4, FP32 walk with
-compensated accumulation, FP64 tree / multipoles / kick-drift. Group size
+batch sums accumulated in FP64, FP64 tree / multipoles / kick-drift. Group size
 256 is the reference driver's default (main.cc:30-33); the target size is
 64 instead of its 32 (measured at this workload: about 4% less step time,
 rms error 2.6e-4 either way; --target-size 32 gives the default), and the
@@ -597,7 +597,7 @@ def main():
             "walk_tile": args.walk_tile,
             "dt": args.dt,
             "seed": args.seed,
-            "walk": "fp64" if args.fp64 else "fp32 + compensated accumulation; fp64 tree, MAC, moments, kick/drift",
+            "walk": "fp64" if args.fp64 else "fp32 interactions, batch sums accumulated in fp64; fp64 tree, MAC, moments, kick/drift",
         },
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "accuracy": acc_report,

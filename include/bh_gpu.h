@@ -77,9 +77,10 @@ enum bh_status {
   BH_ERR_NO_TREE = 7      /* bh_accel / bh_tree_export before bh_build_tree */
 };
 
-/* Walk arithmetic. FP32: interactions in FP32 with compensated per-particle
- * accumulation (default). FP64: interactions in FP64. The MAC, tree build,
- * multipoles and kick/drift are FP64 in both. */
+/* Walk arithmetic. FP32: interactions in FP32, each batch summed in FP32
+ * and the batch sums accumulated per particle in FP64 (default). FP64:
+ * interactions in FP64. The MAC, tree build, multipoles and kick/drift are
+ * FP64 in both. */
 enum bh_precision { BH_FP32 = 0, BH_FP64 = 1 };
 
 /* Node numbering of bh_tree_export: the reference's (backbone BFS, then each
