@@ -39,7 +39,7 @@ struct SysDev {
 };
 static_assert(sizeof(SysDev) == 64, "SysDev layout");
 
-constexpr uint32_t kChunk = 65536;   // particles per bounds block
+constexpr uint32_t kChunk = 4096;    // particles per bounds block
 constexpr uint32_t kMaxSystems = 65535;
 constexpr uint64_t kMaxN = (1ull << 30) - 1;  // particles per step (all systems)
 constexpr uint32_t kGroupCap = 128;  // largest walk tile: a walk unit's particles fill at most two slots per lane

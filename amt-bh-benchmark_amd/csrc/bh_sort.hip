@@ -180,19 +180,40 @@ __global__ __launch_bounds__(kBlock) void k_keys_hist(const Src* __restrict__ p,
   __shared__ uint32_t s_h[8 * 256];
   for (int i = threadIdx.x; i < 8 * 256; i += kBlock) s_h[i] = 0;
   __syncthreads();
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    double c[3], hw;
-    domain_from_lohi(lohi + 6 * sid[i], c, &hw);
-    const double scale = 2097152.0 / (2.0 * hw);
-    const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
-    double x, y, z;
-    bool fin;
-    pos_of(p[i], &x, &y, &z, &fin);
-    const uint64_t key = spread_bits3(quantize(x, lx, scale)) | (spread_bits3(quantize(y, ly, scale)) << 1) |
-                         (spread_bits3(quantize(z, lz, scale)) << 2);
-    keys[i] = key;
+  const int lane = __lane_id();
+  for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += gridDim.x * kBlock) {
+    // the wave's lanes hold consecutive particles; those below n are a
+    // prefix of the wave
+    const uint32_t i = i0 + threadIdx.x;
+    const bool ok = i < n;
+    uint64_t key = 0;
+    if (ok) {
+      double c[3], hw;
+      domain_from_lohi(lohi + 6 * sid[i], c, &hw);
+      const double scale = 2097152.0 / (2.0 * hw);
+      const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
+      double x, y, z;
+      bool fin;
+      pos_of(p[i], &x, &y, &z, &fin);
+      key = spread_bits3(quantize(x, lx, scale)) | (spread_bits3(quantize(y, ly, scale)) << 1) |
+            (spread_bits3(quantize(z, lz, scale)) << 2);
+      keys[i] = key;
+    }
+    // Input in (or near) Morton order, as a resident step's is, gives long
+    // runs of equal digits across the wave, whose atomics on one LDS word
+    // would serialise: the first lane of each run adds the run's length.
+    const uint32_t nvalid = static_cast<uint32_t>(__popcll(__ballot(ok)));
+    const uint64_t prev = __shfl_up(key, 1);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) atomicAdd(&s_h[b * 256 + ((key >> (8 * b)) & 255u)], 1u);
+    for (int b = 0; b < 8; ++b) {
+      const uint32_t d = static_cast<uint32_t>(key >> (8 * b)) & 255u;
+      const uint32_t pd = static_cast<uint32_t>(prev >> (8 * b)) & 255u;
+      const bool head = ok && (lane == 0 || d != pd);
+      const uint64_t heads = __ballot(head);
+      const uint64_t after = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+      const uint32_t end = after ? static_cast<uint32_t>(__ffsll(static_cast<long long>(after)) - 1) : nvalid;
+      if (head) atomicAdd(&s_h[b * 256 + d], end - static_cast<uint32_t>(lane));
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 8 * 256; i += kBlock)
