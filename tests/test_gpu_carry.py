@@ -57,3 +57,30 @@ def test_carried_nonfinite_reported(expose):
         with pytest.raises(bl.BHError) as e:
             ctx.state_step(fp, 16, 256, 32, 0.0, 1e-3)
         assert e.value.code == 2
+
+
+def test_resident_batch_equals_device_batch():
+    """bh_state_step_batch carries the compact positions across a batch of
+    systems too: its leapfrog equals bh_step_batch on two buffers."""
+    specs = [(ic.plummer(3000, 91), 0.5, 16, 64), (ic.hernquist(12000, 92), 0.7, 8, 32),
+             (ic.random_box(700, 93, 2.0, vel_spread=0.2), 0.4, 16, 16), (ic.plummer(20000, 94), 0.6, 32, 128)]
+    p = np.concatenate([q for q, _, _, _ in specs])
+    n = len(p)
+
+    def table(kick, drift):
+        return np.concatenate([layout.system(len(q), layout.force_params(theta, 1e-4, 1.0, True), ml, 256, ts, kick, drift)
+                               for q, theta, ml, ts in specs])
+
+    with bl.Context(0) as ctx:
+        buf = [devmem.DeviceBuffer(n * 64) for _ in range(2)]
+        buf[0].upload(p)
+        cur = 0
+        for kick, drift in SCHED:
+            ctx.step_batch_device(buf[cur].ptr, buf[1 - cur].ptr, table(kick, drift))
+            cur = 1 - cur
+        want = buf[cur].download(np.empty(n, layout.PARTICLE))
+        ctx.state_upload(p)
+        for kick, drift in SCHED:
+            print(f"resident batch kick={kick} drift={drift}", flush=True)
+            ctx.state_step_batch(table(kick, drift))
+        assert golden_io.bytes_equal(ctx.state_download(), want)
