@@ -50,6 +50,16 @@ def test_tree_plummer_and_uniform(gpu, orc):
     check_tree(gpu, orc, ic.uniform_cube(50000, 2), 8, 256)
 
 
+def test_tree_leaves_beyond_one_staging_chunk(gpu, orc):
+    """Leaves whose particles span several of the upward pass's 1,024-particle
+    LDS chunks: a 2,500-fold duplicate (one depth-21 leaf) and max_leaf
+    1,024, the largest allowed; moments bit-exact."""
+    p = ic.random_box(6000, 13, 10.0)
+    p["pos"][100:2600] = p["pos"][5]
+    check_tree(gpu, orc, p, 1024, 4096)
+    check_tree(gpu, orc, ic.plummer(30000, 14), 1024, 8192)
+
+
 def test_tree_duplicates_and_degenerate(gpu, orc):
     p = ic.random_box(3000, 3, 10.0)
     p["pos"][:1000] = p["pos"][17]
