@@ -114,22 +114,10 @@ __device__ inline double fmaR(double a, double b, double c) { return fma(a, b, c
 template <typename Real>
 struct Acc {
   double x = 0.0, y = 0.0, z = 0.0;
-  __device__ inline void add(float bx, float by, float bz) {
+  __device__ inline void add(Real bx, Real by, Real bz) {
     x += static_cast<double>(bx);
     y += static_cast<double>(by);
     z += static_cast<double>(bz);
-  }
-  __device__ inline double sx() const { return x; }
-  __device__ inline double sy() const { return y; }
-  __device__ inline double sz() const { return z; }
-};
-template <>
-struct Acc<double> {
-  double x = 0.0, y = 0.0, z = 0.0;
-  __device__ inline void add(double bx, double by, double bz) {
-    x += bx;
-    y += by;
-    z += bz;
   }
   __device__ inline double sx() const { return x; }
   __device__ inline double sy() const { return y; }

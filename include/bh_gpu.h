@@ -254,7 +254,9 @@ int bh_server_pending(const bh_server* server, uint32_t* pending, uint32_t* acti
 double bh_server_clock(const bh_server* server);
 
 /* Particles resident in device memory across steps (the caller's
- * GroupedWorkspace reuse, main.cc:203-242, without host copies). */
+ * GroupedWorkspace reuse, main.cc:203-242, without host copies). A resident
+ * step's walk also keeps the new positions in a compact array, from which
+ * the next resident step builds its tree when nothing else ran in between. */
 int bh_state_upload(bh_ctx* ctx, const bh_particle* host, uint64_t n);
 int bh_state_step(bh_ctx* ctx, const bh_force_params* fp, uint32_t max_leaf, uint32_t group_size, uint32_t target_size,
                   double kick_dt, double drift_dt, bh_step_timing* timing);
@@ -262,7 +264,9 @@ int bh_state_step(bh_ctx* ctx, const bh_force_params* fp, uint32_t max_leaf, uin
  * to the resident count). */
 int bh_state_step_batch(bh_ctx* ctx, const bh_system* systems, uint32_t n_systems, bh_step_timing* timing);
 int bh_state_download(bh_ctx* ctx, bh_particle* host, uint64_t n);
-/* Device address of the resident particles (Morton order after a step). */
+/* Device address of the resident particles (Morton order after a step).
+ * The caller may write through it: the next resident step then reads the
+ * particle records again. */
 bh_particle* bh_state_device_ptr(bh_ctx* ctx);
 
 /* build_flat_tree (flat_build_ttg.h:276-281) of host particles; the tree stays
