@@ -857,12 +857,12 @@ __device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax
   }
 }
 
-// The largest target the group evaluator's layouts take (its slots hold
-// up to 64 particles in every layout); smaller targets use lane = (particle,
+// Targets the group evaluator takes (its layouts cover 33 to kGroupCap
+// particles, two slots per lane); smaller targets use lane = (particle,
 // slice), larger ones walk in chunks of 64 particles.
-constexpr uint32_t kWideMin = 33, kWideMax = kWave;
+constexpr uint32_t kWideMin = 33, kWideMax = kGroupCap;
 
-// One target (node tnode) of at most 32 or more than 64 particles: the
+// One target (node tnode) of at most 32 or more than kGroupCap particles: the
 // walk over `start` (the group's deferred nodes) or from the root, for each
 // chunk of <= 64 of its particles; then the slices' sums, plus the group's
 // share (gacc, counts g_pp / g_pc) when given.
@@ -939,10 +939,11 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
 //
 // The unit's work is a sequence of jobs: the group's shared walk (if any),
 // then its targets in particle order. The group's walk and the targets of
-// 33-64 particles use the group evaluator (its layouts keep the lanes
-// busy for those sizes) through one call of its traversal; the other
-// targets use the target evaluator. Deferral is off for target jobs
-// (no deferred list): their decisions are accept or open.
+// 33-128 particles use the group evaluator (its layouts keep the lanes
+// busy for those sizes, and a target of 65-128 particles walks once, not
+// once per 64) through one call of its traversal; the other targets use the
+// target evaluator. Deferral is off for target jobs (no deferred list):
+// their decisions are accept or open.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g) {
   const int lane = __lane_id();
@@ -995,7 +996,7 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g)
       walk_target<Real, kMask>(a, L, gfr, tnode, start, n_start, gacc, jpp, jpc, tot_pp, tot_pc);
       continue;
     }
-    // group evaluator: the group's shared walk, or one target of 33-64 particles
+    // group evaluator: the group's shared walk, or one target of 33-128 particles
     const uint32_t b0 = is_group ? gb : uniform(a.begin[tnode]);
     const uint32_t size = is_group ? G : T;
     // a packed unit's box and the origin of its relative coordinates (the

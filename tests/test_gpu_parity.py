@@ -111,26 +111,28 @@ def test_walk_parity(gpu, orc, n, ml, gs, ts, theta, eps2, quad):
 
 
 @pytest.mark.parametrize("mode", [bl.BH_WALK_GROUPED, bl.BH_WALK_PER_TARGET])
-def test_wide_targets(gpu, orc, mode):
-    """Targets of 33-64 particles go through the group evaluator's layouts
-    (two slots for 33-42, 32 + the rest for 43-53, one slot for 54-64);
-    smaller and larger ones through the target evaluator. Target size 64
-    over leaves of 16 gives all of them; lists exact, sums within
-    tolerance, in both walk schedules."""
+@pytest.mark.parametrize("ts,ranges", [(64, ((1, 32), (33, 42), (43, 53), (54, 64))),
+                                       (128, ((1, 32), (33, 64), (65, 96), (97, 128)))])
+def test_wide_targets(gpu, orc, mode, ts, ranges):
+    """Targets of 33-128 particles go through the group evaluator's layouts
+    (two slots for 33-42, 32 + the rest for 43-53, one slot for 54-64, 64 +
+    the rest for 65-96, two full slots for 97-128); smaller ones through the
+    target evaluator. Target sizes 64 and 128 over leaves of 16 give all of
+    them; lists exact, sums within tolerance, in both walk schedules."""
     p = ic.plummer(40000, 23)
     fp = layout.force_params(0.5, 1e-4, 1.0, True)
     ot = orc.build(p, 16, 256)
-    sizes = ot.export()["nodes"]["particle_count"][ot.targets(64)]
-    for lo, hi in ((1, 32), (33, 42), (43, 53), (54, 64)):
+    sizes = ot.export()["nodes"]["particle_count"][ot.targets(ts)]
+    for lo, hi in ranges:
         assert ((sizes >= lo) & (sizes <= hi)).any(), (lo, hi)
-    oa, opp, opc, otot = ot.grouped_pass(fp, 64)
+    oa, opp, opc, otot = ot.grouped_pass(fp, ts)
     gpu.set_walk_mode(mode)
     try:
         gpu.build_tree(p, 16, 256)
         for prec in (bl.BH_FP64, bl.BH_FP32):
-            print(f"wide targets mode={mode} prec={prec}", flush=True)
+            print(f"wide targets ts={ts} mode={mode} prec={prec}", flush=True)
             gpu.set_precision(prec)
-            ga, gpp, gpc, gtot = gpu.accel(fp, 64)
+            ga, gpp, gpc, gtot = gpu.accel(fp, ts)
             assert np.array_equal(gpp, opp) and np.array_equal(gpc, opc) and np.array_equal(gtot, otot)
             assert rel_err(ga, oa) < TOL[prec]
     finally:
