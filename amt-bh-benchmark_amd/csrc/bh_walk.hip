@@ -857,15 +857,17 @@ __device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax
   }
 }
 
-// Targets the group evaluator takes (its layouts cover 33 to kGroupCap
-// particles, two slots per lane); smaller targets use lane = (particle,
-// slice), larger ones walk in chunks of 64 particles.
-constexpr uint32_t kWideMin = 33, kWideMax = kGroupCap;
+// Targets of at least kWideMin particles use the group evaluator (its
+// layouts cover 33 to kGroupCap particles, two slots per lane; larger
+// targets walk in chunks of kGroupCap); smaller ones use lane = (particle,
+// slice).
+constexpr uint32_t kWideMin = 33;
 
-// One target (node tnode) of at most 32 or more than kGroupCap particles: the
-// walk over `start` (the group's deferred nodes) or from the root, for each
-// chunk of <= 64 of its particles; then the slices' sums, plus the group's
-// share (gacc, counts g_pp / g_pc) when given.
+// One target (node tnode) with the target evaluator (the unit's loop gives
+// it targets of at most 32 particles): the walk over `start` (the group's
+// deferred nodes) or from the root, for each chunk of <= 64 of its
+// particles; then the slices' sums, plus the group's share (gacc, counts
+// g_pp / g_pc) when given.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t tnode, const uint32_t* start,
                             uint32_t n_start, const double* gacc, uint32_t g_pp, uint32_t g_pc,
@@ -938,12 +940,13 @@ __device__ void walk_target(const WalkArgs& a, Lds& L, uint32_t* gfr, uint32_t t
 // unit walks only those.
 //
 // The unit's work is a sequence of jobs: the group's shared walk (if any),
-// then its targets in particle order. The group's walk and the targets of
-// 33-128 particles use the group evaluator (its layouts keep the lanes
-// busy for those sizes, and a target of 65-128 particles walks once, not
-// once per 64) through one call of its traversal; the other targets use the
-// target evaluator. Deferral is off for target jobs (no deferred list):
-// their decisions are accept or open.
+// then its targets in particle order (a target of more than 128 particles
+// as jobs of 128 of them, each walking the target's list with its box). The
+// group's walk and the targets of 33 particles or more use the group
+// evaluator (its layouts keep the lanes busy for those sizes, and a target
+// of 65-128 particles walks once, not once per 64) through one call of its
+// traversal; smaller targets use the target evaluator. Deferral is off for
+// target jobs (no deferred list): their decisions are accept or open.
 template <typename Real, bool kMask, typename Lds>
 __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g) {
   const int lane = __lane_id();
@@ -965,13 +968,18 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g)
   uint32_t c = 0;     // next 64-particle chunk of the unit to scan for targets
   uint64_t bits = 0;  // targets of the scanned chunk not yet walked
   uint32_t m = 0;     // this lane's mark in the scanned chunk
+  uint32_t big = 0, big_T = 0, big_off = 0;  // a target of > kGroupCap particles, walked in chunks
   for (;;) {
-    // the next job: the group's walk, or a target node
+    // the next job: the group's walk, a target node, or the next chunk of a
+    // large target
     bool is_group = false;
-    uint32_t tnode = g;
+    uint32_t tnode = g, off = 0;
     if (group_job) {
       is_group = true;
       group_job = false;
+    } else if (big_off < big_T) {
+      tnode = big;
+      off = big_off;
     } else if (single_job) {
       single_job = false;
     } else {
@@ -992,13 +1000,21 @@ __device__ void walk_group(const WalkArgs& a, Lds& L, uint32_t slot, uint32_t g)
     const double* gacc = ok ? a.gacc : nullptr;
     const uint32_t jpp = ok ? g_pp : 0u, jpc = ok ? g_pc : 0u;
     const uint32_t T = is_group ? 0u : uniform(a.count[tnode]);
-    if (!is_group && !(T >= kWideMin && T <= kWideMax)) {
+    if (!is_group && T < kWideMin) {
       walk_target<Real, kMask>(a, L, gfr, tnode, start, n_start, gacc, jpp, jpc, tot_pp, tot_pc);
       continue;
     }
-    // group evaluator: the group's shared walk, or one target of 33-128 particles
-    const uint32_t b0 = is_group ? gb : uniform(a.begin[tnode]);
-    const uint32_t size = is_group ? G : T;
+    if (!is_group && T > kGroupCap && off == 0 && big_T == 0) {
+      big = tnode;  // its chunks follow, this one first
+      big_T = T;
+    }
+    if (!is_group && T > kGroupCap) big_off = off + kGroupCap;
+    if (!is_group && big_T != 0 && big_off >= big_T) big_T = big_off = 0;
+    // group evaluator: the group's shared walk, a target of 33-128 particles
+    // or a chunk of up to 128 of a larger target (its box and list, its
+    // chunk's particles)
+    const uint32_t b0 = is_group ? gb : uniform(a.begin[tnode]) + off;
+    const uint32_t size = is_group ? G : (T - off < kGroupCap ? T - off : kGroupCap);
     // a packed unit's box and the origin of its relative coordinates (the
     // box centre); a group's or target's: its node box and centre
     const bool pbox = is_group && packed;
