@@ -3,8 +3,8 @@
 Bars: keys, permutation, sorted particles, node arrays, centres, second
 moments and group roots bit-exact (reference node layout); per-particle
 interaction-list lengths exact (so the MAC decisions are identical);
-accelerations within 1e-12 (FP64 walk) and 1e-5 (FP32 walk with compensated
-accumulation), global-normalised, against the FP64 oracle.
+accelerations within 1e-12 (FP64 walk) and 1e-5 (FP32 walk, batch sums
+accumulated in FP64), global-normalised, against the FP64 oracle.
 """
 import numpy as np
 import pytest
