@@ -626,12 +626,22 @@ int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_
   const SysDev* sys = reinterpret_cast<const SysDev*>(b.tables);
   const uint64_t lb_words = radix_lookback_words(n);
   HIP_OK(c, hipMemcpyAsync(b.tables, c->host_tables, p.table_bytes, hipMemcpyHostToDevice, s));
-  HIP_OK(c, launch_fill_u32(b.misc, 0u, kMiscWords, s));
-  HIP_OK(c, launch_fill_u32(reinterpret_cast<uint32_t*>(b.totals), 0u, 4, s));
-  HIP_OK(c, launch_fill_u32(b.hist, 0u, 8 * 256, s));
+  {
+    // the step's flag words, totals, key digit histograms and look-back
+    // words, in one launch
+    ZeroRanges z;
+    z.p[0] = b.misc;
+    z.n[0] = kMiscWords;
+    z.p[1] = reinterpret_cast<uint32_t*>(b.totals);
+    z.n[1] = 4;
+    z.p[2] = b.hist;
+    z.n[2] = 8 * 256;
+    z.p[3] = b.lookback;
+    z.n[3] = lb_words;
+    HIP_OK(c, launch_zero(z, s));
+  }
   if (p.sid_passes)
     HIP_OK(c, hipMemcpyAsync(b.hist + 8 * 256, c->host_tables + p.off_hist, 2 * 256 * 4, hipMemcpyHostToDevice, s));
-  HIP_OK(c, launch_fill_u32(b.lookback, 0u, lb_words, s));
   HIP_OK(c, launch_fill_sid(chunks, p.n_chunks, b.sid, s));
   if (p.carried_in) {
     HIP_OK(c, launch_bounds(c->carry.p, c->carry_flag.p, chunks, p.n_chunks, chunk_first, p.S, b.partial,
@@ -643,7 +653,10 @@ int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_
   }
   HIP_OK(c, launch_radix_sort(b.keys_a, b.vals_a, b.keys_b, b.vals_b, n, p.sid_passes, b.sid, b.hist, b.dbase,
                               b.lookback, b.lookback + (lb_words - 10), &out->keys, &out->perm, s));
-  HIP_OK(c, launch_gather(d_in, walk_writes_out ? nullptr : d_out, out->perm, b.posm, n, s));
+  if (p.carried_in && walk_writes_out)
+    HIP_OK(c, launch_gather(c->carry.p, out->perm, b.posm, n, s));
+  else
+    HIP_OK(c, launch_gather(d_in, walk_writes_out ? nullptr : d_out, out->perm, b.posm, n, s));
   if (events) HIP_OK(c, hipEventRecord(c->ev[1], s));
   HIP_OK(c, launch_tree_count(out->keys, n, b.sid, sys, p.mlx, b.L, b.H, b.tcounts, s));
   HIP_OK(c, launch_tree_scan(b.tcounts, tiles, b.tprefix, b.misc + kMiscLevels, s));

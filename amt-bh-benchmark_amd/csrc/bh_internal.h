@@ -135,6 +135,14 @@ struct WalkArgs {
   unsigned long long* totals;  // [2]
 };
 
+// Up to kMax device ranges of 32-bit words to zero (unused: n = 0).
+struct ZeroRanges {
+  static constexpr int kMax = 4;
+  uint32_t* p[kMax] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t n[kMax] = {0, 0, 0, 0};
+};
+hipError_t launch_zero(const ZeroRanges& r, hipStream_t s);
+
 // ---- bh_sort.hip
 // chunks[c] = {system, begin, end, 0}: blocks of <= kChunk particles of one
 // system; chunk_first[s] = first chunk of system s (chunk_first[S] = total).
@@ -164,6 +172,9 @@ hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_
 // Morton-ordered particles (a step's walk writes those itself).
 hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
                          hipStream_t s);
+// The same walk copy from a carried compact copy (x, y, z, m) in the order
+// perm indexes.
+hipError_t launch_gather(const double4* pos, const uint32_t* perm, double4* posm, uint32_t n, hipStream_t s);
 uint64_t radix_lookback_words(uint32_t n);  // 10 passes of look-back words + 10 tile counters
 
 // ---- bh_tree.hip

@@ -67,6 +67,16 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t v, ui
   for (uint64_t i = blockIdx.x * uint64_t(kBlock) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) p[i] = v;
 }
 
+// Zeroes several ranges in one launch (a step's flag words, totals, digit
+// histograms and look-back words).
+__global__ __launch_bounds__(kBlock) void k_zero_ranges(ZeroRanges r) {
+  const uint64_t stride = uint64_t(gridDim.x) * kBlock;
+  const uint64_t t = blockIdx.x * uint64_t(kBlock) + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < ZeroRanges::kMax; ++k)
+    for (uint64_t i = t; i < r.n[k]; i += stride) r.p[k][i] = 0u;
+}
+
 __device__ inline double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
   return v;
@@ -111,17 +121,28 @@ __global__ __launch_bounds__(kBlock) void k_bounds_partial(const Src* __restrict
   bool bad = false;
   if (carried_bad && blockIdx.x == 0 && threadIdx.x == 0 && *carried_bad) atomicOr(nonfinite, 1u);
   const uint4 c = chunks[blockIdx.x];
-  for (uint32_t i = c.y + threadIdx.x; i < c.z; i += kBlock) {
-    double x, y, z;
-    bool fin;
-    pos_of(p[i], &x, &y, &z, &fin);
-    bad |= !fin;
-    lo[0] = fmin(lo[0], x);
-    lo[1] = fmin(lo[1], y);
-    lo[2] = fmin(lo[2], z);
-    hi[0] = fmax(hi[0], x);
-    hi[1] = fmax(hi[1], y);
-    hi[2] = fmax(hi[2], z);
+  // kB elements per lane and round, all loads issued before the first use
+  // (indices clamped into the chunk, extra elements dropped after the load)
+  constexpr int kB = 4;
+  for (uint32_t i0 = c.y + threadIdx.x; i0 < c.z; i0 += kB * kBlock) {
+    double x[kB], y[kB], z[kB];
+    bool fin[kB];
+#pragma unroll
+    for (int r = 0; r < kB; ++r) {
+      const uint32_t i = i0 + r * kBlock < c.z ? i0 + r * kBlock : c.z - 1;
+      pos_of(p[i], &x[r], &y[r], &z[r], &fin[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < kB; ++r) {
+      if (i0 + r * kBlock >= c.z) break;
+      bad |= !fin[r];
+      lo[0] = fmin(lo[0], x[r]);
+      lo[1] = fmin(lo[1], y[r]);
+      lo[2] = fmin(lo[2], z[r]);
+      hi[0] = fmax(hi[0], x[r]);
+      hi[1] = fmax(hi[1], y[r]);
+      hi[2] = fmax(hi[2], z[r]);
+    }
   }
   __shared__ double s[4][6];
   const int w = threadIdx.x >> 6, lane = __lane_id();
@@ -181,20 +202,37 @@ __global__ __launch_bounds__(kBlock) void k_keys_hist(const Src* __restrict__ p,
   for (int i = threadIdx.x; i < 8 * 256; i += kBlock) s_h[i] = 0;
   __syncthreads();
   const int lane = __lane_id();
-  for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += gridDim.x * kBlock) {
+  const uint32_t stride = gridDim.x * kBlock;
+  // each particle's position and system are loaded one round ahead, so the
+  // loads of round r + 1 are in flight while round r computes
+  double nx = 0.0, ny = 0.0, nz = 0.0;
+  uint32_t ns = 0;
+  {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+      bool fin;
+      pos_of(p[i], &nx, &ny, &nz, &fin);
+      ns = sid[i];
+    }
+  }
+  for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += stride) {
     // the wave's lanes hold consecutive particles; those below n are a
     // prefix of the wave
     const uint32_t i = i0 + threadIdx.x;
     const bool ok = i < n;
+    const double x = nx, y = ny, z = nz;
+    const uint32_t sys_i = ns;
+    if (i + stride < n) {
+      bool fin;
+      pos_of(p[i + stride], &nx, &ny, &nz, &fin);
+      ns = sid[i + stride];
+    }
     uint64_t key = 0;
     if (ok) {
       double c[3], hw;
-      domain_from_lohi(lohi + 6 * sid[i], c, &hw);
+      domain_from_lohi(lohi + 6 * sys_i, c, &hw);
       const double scale = 2097152.0 / (2.0 * hw);
       const double lx = c[0] - hw, ly = c[1] - hw, lz = c[2] - hw;
-      double x, y, z;
-      bool fin;
-      pos_of(p[i], &x, &y, &z, &fin);
       key = spread_bits3(quantize(x, lx, scale)) | (spread_bits3(quantize(y, ly, scale)) << 1) |
             (spread_bits3(quantize(z, lz, scale)) << 2);
       keys[i] = key;
@@ -400,6 +438,19 @@ __global__ __launch_bounds__(kBlock) void k_gather(const bh_particle* __restrict
   if (part == 2) q[1] = make_uint4(v.x, v.y, mlo, mhi);
 }
 
+// The walk copy from a carried compact copy (x, y, z, m in the last resident
+// step's order, which is the order perm indexes): two lanes per particle, one
+// 16-byte half each. The copy holds the same FP64 values as the records, so
+// the result equals k_gather's.
+__global__ __launch_bounds__(kBlock) void k_gather_pos(const double4* __restrict__ pos, const uint32_t* __restrict__ perm,
+                                                       double4* __restrict__ posm, uint32_t n) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t k = t >> 1, half = t & 1u;
+  if (k >= n) return;
+  const uint32_t j = perm[k];
+  reinterpret_cast<uint4*>(posm + k)[half] = reinterpret_cast<const uint4*>(pos + j)[half];
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t cap) {
   uint64_t g = (n + kBlock - 1) / kBlock;
   if (g < 1) g = 1;
@@ -411,6 +462,14 @@ inline uint32_t grid_for(uint64_t n, uint32_t cap) {
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 4096)), dim3(kBlock), 0, s, p, value, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(const ZeroRanges& r, hipStream_t s) {
+  uint64_t most = 0;
+  for (int k = 0; k < ZeroRanges::kMax; ++k) most = r.n[k] > most ? r.n[k] : most;
+  if (most == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_ranges, dim3(grid_for(most, 4096)), dim3(kBlock), 0, s, r);
   return hipGetLastError();
 }
 
@@ -492,6 +551,11 @@ hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_
 hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
                          hipStream_t s) {
   hipLaunchKernelGGL(k_gather, dim3((4ull * n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, in, out, perm, posm, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const double4* pos, const uint32_t* perm, double4* posm, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_pos, dim3((2ull * n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, perm, posm, n);
   return hipGetLastError();
 }
 

@@ -47,6 +47,23 @@ __device__ inline uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// OR of v over the wavefront (DPP inclusive scan, lane 63), wave-uniform.
+__device__ inline uint32_t wave_or(uint32_t v) {
+  int x = static_cast<int>(v);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(x, 63));
+}
+
+// Levels lj..hj as bits (0 when the particle begins no node).
+__device__ inline uint32_t level_bits(int lj, int hj) {
+  return lj <= hj ? (2u << hj) - (1u << lj) : 0u;
+}
+
 // Number of leading octal digits (of 21) two 63-bit keys share.
 __device__ inline int common_digits(uint64_t a, uint64_t b) {
   const uint64_t x = a ^ b;
@@ -84,10 +101,23 @@ __global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restric
   __shared__ uint32_t s_one;  // 1 + system id when the tile and its halo hold one system, else 0
   const uint32_t tile = blockIdx.x, base = tile * kTreeTile;
   const int64_t g0 = static_cast<int64_t>(base) - mlx - 1;  // global index of s_k[0]
-  const uint32_t wk = kTreeTile + 2 * mlx + 1;
-  for (uint32_t e = threadIdx.x; e < wk; e += kBlock) {
-    const int64_t g = g0 + e;
-    s_k[e] = (g >= 0 && g < n) ? keys[g] : 0ull;
+  constexpr int kPer = static_cast<int>(kTreeTile) / kBlock;
+  {
+    // the tile's keys: every load in flight before the first LDS write
+    uint64_t kv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const uint32_t i = base + threadIdx.x + j * kBlock;
+      kv[j] = i < n ? keys[i] : 0ull;
+    }
+    // the halos [g0, base) and [base + tile, base + tile + mlx)
+    for (uint32_t h = threadIdx.x; h < 2 * mlx + 1; h += kBlock) {
+      const uint32_t e = h <= mlx ? h : h + kTreeTile;
+      const int64_t g = g0 + e;
+      s_k[e] = (g >= 0 && g < n) ? keys[g] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) s_k[mlx + 1 + threadIdx.x + j * kBlock] = kv[j];
   }
   if (threadIdx.x < kTreeCols) s_cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
@@ -97,27 +127,91 @@ __global__ __launch_bounds__(kBlock) void k_tree_count(const uint64_t* __restric
   }
   __syncthreads();
   const uint32_t one = s_one;
+  // the system's off, n and max_leaf: loaded once when the tile holds one
+  // system, per particle otherwise
+  uint32_t o_off = 0, o_n = 0, o_ml = 0;
+  if (one) {
+    o_off = sys[one - 1u].off;
+    o_n = sys[one - 1u].n;
+    o_ml = sys[one - 1u].max_leaf;
+  }
   // P_u for u in [base - mlx, base + tile)
   for (uint32_t e = threadIdx.x; e < kTreeTile + mlx; e += kBlock) {
     const int64_t u = static_cast<int64_t>(base) - mlx + e;
     int8_t pv = -1;
     if (u >= 0 && u < n) {
-      const SysDev& q = sys[one ? one - 1u : sid[u]];
-      if (u + q.max_leaf < q.off + q.n) pv = static_cast<int8_t>(common_digits(s_k[u - g0], s_k[u + q.max_leaf - g0]));
+      uint32_t qoff = o_off, qn = o_n, qml = o_ml;
+      if (!one) {
+        const SysDev& q = sys[sid[u]];
+        qoff = q.off;
+        qn = q.n;
+        qml = q.max_leaf;
+      }
+      if (u + qml < qoff + qn) pv = static_cast<int8_t>(common_digits(s_k[u - g0], s_k[u + qml - g0]));
     }
     s_p[e] = pv;
   }
+  // L_i while the keys are still here (their space holds the window table
+  // below)
+  int li_r[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t e = threadIdx.x + j * kBlock, i = base + e;
+    li_r[j] = 0;
+    if (i < n) {
+      const uint32_t qo = one ? o_off : sys[sid[i]].off;
+      li_r[j] = (i == qo) ? 0 : common_digits(s_k[i - 1 - g0], s_k[i - g0]) + 1;
+    }
+  }
   __syncthreads();
-  for (uint32_t e = threadIdx.x; e < kTreeTile; e += kBlock) {
-    const uint32_t i = base + e;
+  // W_i = max P over the window [i - span, i]. With one system in the tile
+  // and its halo, full windows (span = max_leaf, w = max_leaf + 1 entries)
+  // come from a doubling table: level k holds max P over [e, e + 2^k), and
+  // with 2^K <= w < 2^(K+1) a window is the max of two level-K entries.
+  // Levels are built in two byte buffers over the keys' space; level 0 is
+  // s_p itself, kept for the windows cut short at a system's first
+  // particles and for tiles that hold several systems (direct scan).
+  const uint32_t np = kTreeTile + mlx;
+  int8_t* const tb0 = reinterpret_cast<int8_t*>(s_k);
+  int8_t* const tb1 = tb0 + ((np + 15u) & ~15u);
+  const int8_t* top = s_p;
+  int K = 0;
+  if (one) {
+    const uint32_t w = o_ml + 1u;
+    K = 31 - __clz(static_cast<int>(w));
+    for (int k = 1; k <= K; ++k) {
+      int8_t* dst = (k & 1) ? tb0 : tb1;
+      const uint32_t h = 1u << (k - 1);
+      for (uint32_t e = threadIdx.x; e < np; e += kBlock) {
+        const int a0 = top[e];
+        const int a1 = e + h < np ? top[e + h] : -1;
+        dst[e] = static_cast<int8_t>(a0 > a1 ? a0 : a1);
+      }
+      __syncthreads();
+      top = dst;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t e = threadIdx.x + j * kBlock, i = base + e;
     if (i >= n) break;
-    const SysDev& q = sys[one ? one - 1u : sid[i]];
-    const uint32_t ml = q.max_leaf;
-    const int li = (i == q.off) ? 0 : common_digits(s_k[i - 1 - g0], s_k[i - g0]) + 1;
-    const uint32_t span = (i - q.off < ml) ? i - q.off : ml;  // u = i - d, d <= span, stays in the system
+    uint32_t qoff = o_off, qn = o_n, ml = o_ml;
+    if (!one) {
+      const SysDev& q = sys[sid[i]];
+      qoff = q.off;
+      qn = q.n;
+      ml = q.max_leaf;
+    }
+    const int li = li_r[j];
+    const uint32_t span = (i - qoff < ml) ? i - qoff : ml;  // u = i - d, d <= span, stays in the system
     int wmax = -1;
-    for (uint32_t d = 0; d <= span; ++d) wmax = max(wmax, static_cast<int>(s_p[e + mlx - d]));
-    const int hi = (q.n > ml) ? min(wmax, kDepthCap - 1) + 1 : 0;
+    if (one && span == ml) {
+      const int a0 = top[e + mlx - ml], a1 = top[e + mlx + 1u - (1u << K)];
+      wmax = a0 > a1 ? a0 : a1;
+    } else {
+      for (uint32_t d = 0; d <= span; ++d) wmax = max(wmax, static_cast<int>(s_p[e + mlx - d]));
+    }
+    const int hi = (qn > ml) ? min(wmax, kDepthCap - 1) + 1 : 0;
     L[i] = static_cast<uint8_t>(li);
     H[i] = static_cast<uint8_t>(hi);
     for (int d = li; d <= hi; ++d) atomicAdd(&s_cnt[d], 1u);
@@ -194,13 +288,23 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit(const uint8_t* __restrict_
   uint32_t c[kLevels + 1];
 #pragma unroll
   for (int d = 0; d <= kLevels; ++d) c[d] = 0;
+  // an item's L and H, packed (255 = no particle); the next item's load is
+  // issued before the current item's levels are counted
+  auto load_lh = [&](int j) -> uint32_t {
+    const uint32_t idx = base + w * (kTreeTile / 4) + j * 64 + lane;
+    return idx < n ? static_cast<uint32_t>(L[idx]) | (static_cast<uint32_t>(H[idx]) << 8) : 255u;
+  };
+  uint32_t nlh = load_lh(0);
 #pragma unroll 1
   for (int j = 0; j < kItems; ++j) {
-    const uint32_t idx = base + w * (kTreeTile / 4) + j * 64 + lane;
-    const bool ok = idx < n;
-    const int lj = ok ? L[idx] : 255, hj = ok ? H[idx] : 0;
+    const uint32_t lh = nlh;
+    if (j + 1 < kItems) nlh = load_lh(j + 1);
+    const int lj = static_cast<int>(lh & 255u), hj = static_cast<int>(lh >> 8);
+    // only the levels where some lane begins a node
+    const uint32_t wm = wave_or(level_bits(lj, hj));
 #pragma unroll
-    for (int d = 0; d < kLevels; ++d) c[d] += __popcll(__ballot(lj <= d && d <= hj));
+    for (int d = 0; d < kLevels; ++d)
+      if ((wm >> d) & 1u) c[d] += __popcll(__ballot(lj <= d && d <= hj));
     c[kLevels] += __popcll(__ballot(lj <= hj));
   }
   if (lane == 0) {
@@ -224,22 +328,32 @@ __global__ __launch_bounds__(kBlock) void k_tree_emit(const uint8_t* __restrict_
   run[kLevels] = tile_prefix[uint64_t(tile) * kTreeCols + kLevels] + s_wc[w][kLevels];
   const uint32_t total_nodes = s_loff[kLevels];
   const uint64_t lt = lanemask_lt();
+  nlh = load_lh(0);
 #pragma unroll 1
   for (int j = 0; j < kItems; ++j) {
     const uint32_t idx = base + w * (kTreeTile / 4) + j * 64 + lane;
-    const bool ok = idx < n;
-    const int lj = ok ? L[idx] : 255, hj = ok ? H[idx] : 0;
+    const uint32_t lh = nlh;
+    if (j + 1 < kItems) nlh = load_lh(j + 1);
+    const int lj = static_cast<int>(lh & 255u), hj = static_cast<int>(lh >> 8);
     const uint64_t lm = __ballot(lj <= hj);
     const uint32_t lr = run[kLevels] + __popcll(lm & lt);
     run[kLevels] += __popcll(lm);
+    // levels with begins in this wave, and the levels just below them (the
+    // children's start ranks); elsewhere no lane begins a node and a rank is
+    // the running count
+    const uint32_t wm = wave_or(level_bits(lj, hj));
+    const uint32_t need = wm | (wm << 1);
     uint32_t prev = 0;
 #pragma unroll
     for (int d = 0; d <= kLevels; ++d) {
       uint32_t rank;
       if (d < kLevels) {
-        const uint64_t m = __ballot(lj <= d && d <= hj);
-        rank = run[d] + __popcll(m & lt);
-        run[d] += __popcll(m);
+        rank = run[d];
+        if ((need >> d) & 1u) {
+          const uint64_t m = __ballot(lj <= d && d <= hj);
+          rank += __popcll(m & lt);
+          run[d] += __popcll(m);
+        }
       } else {
         rank = total_nodes;
       }
@@ -384,12 +498,25 @@ __global__ __launch_bounds__(kBlock) void k_upward_leaves(const double4* __restr
     for (uint32_t c0 = span_lo; c0 < span_hi; c0 += kUpChunk) {
       const uint32_t cn = span_hi - c0 < kUpChunk ? span_hi - c0 : kUpChunk;
       __syncthreads();
-      for (uint32_t k = threadIdx.x; k < cn; k += kBlock) {
-        const double4 q = posm[c0 + k];
-        s_x[k] = q.x;
-        s_y[k] = q.y;
-        s_z[k] = q.z;
-        s_m[k] = q.w;
+      {
+        // every load of the chunk in flight before the first LDS write
+        constexpr int kR = kUpChunk / kBlock;
+        double4 q[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const uint32_t k = threadIdx.x + r * kBlock;
+          q[r] = posm[c0 + (k < cn ? k : cn - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          const uint32_t k = threadIdx.x + r * kBlock;
+          if (k < cn) {
+            s_x[k] = q[r].x;
+            s_y[k] = q[r].y;
+            s_z[k] = q[r].z;
+            s_m[k] = q[r].w;
+          }
+        }
       }
       __syncthreads();
       const uint32_t lo = b > c0 ? b : c0;
