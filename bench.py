@@ -5,8 +5,9 @@ sphere N = 16,777,216 (G = M = a = 1, r < 10a, seed 3), eps = 1e-3,
 theta = 0.5, leaf capacity 32, leapfrog dt = 5e-4, FP32 walk with
 batch sums accumulated in FP64, FP64 tree / multipoles / kick-drift. Group size
 256 is the reference driver's default (main.cc:30-33); the target size is
-64 instead of its 32 (measured at this workload: about 4% less step time,
-rms error 2.6e-4 either way; --target-size 32 gives the default), and the
+96 instead of its 32 (measured at this workload: the fastest of the sizes
+32 to 128, rms error 2.6e-4 or below for all; --target-size 32 gives the
+default), and the
 walk tile is the largest, 128 particles per walk unit.
 
 A step is one step_particles call (tree build + walk + fused kick/drift) on
@@ -472,7 +473,7 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-4)
     ap.add_argument("--max-leaf", type=int, default=32)
     ap.add_argument("--group-size", type=int, default=256)
-    ap.add_argument("--target-size", type=int, default=64)
+    ap.add_argument("--target-size", type=int, default=96)
     ap.add_argument("--walk-tile", type=int, default=128, help="particles per walk unit (16-128)")
     ap.add_argument("--fp64", action="store_true", help="FP64 walk arithmetic")
     ap.add_argument("--error-subset", type=int, default=4096)

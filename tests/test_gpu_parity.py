@@ -37,7 +37,10 @@ def check_tree(gpu, orc, p, ml, gs):
 
 CASES = [(1, 8, 64, 1), (2, 8, 64, 1), (9, 8, 64, 2), (200, 4, 16, 1), (2000, 8, 64, 4), (4096, 16, 256, 3),
          (4097, 16, 256, 3), (20000, 16, 64, 5), (20000, 16, 8192, 5), (5000, 1, 64, 6), (3000, 0, 64, 7),
-         (70000, 32, 256, 9)]
+         (70000, 32, 256, 9),
+         # leaf capacities whose windows (max_leaf + 1) need 3, 6, 6 and 8
+         # doubling levels in the count kernel's window table (7: exactly 8)
+         (30000, 7, 64, 10), (30000, 63, 256, 11), (40000, 100, 512, 12), (40000, 300, 1024, 13)]
 
 
 @pytest.mark.parametrize("n,ml,gs,seed", CASES)
