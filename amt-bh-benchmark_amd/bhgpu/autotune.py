@@ -43,7 +43,7 @@ MAX_TILE = 128
 DEFAULT = (16, 256, 32, MAX_TILE)  # (max_leaf, group_size, target_size, walk_tile)
 LEAVES = (8, 16, 32, 64)
 GROUPS = (64, 128, 256)
-TARGETS = (16, 32, 64, 128, 256)
+TARGETS = (16, 32, 64, 96, 128, 256)
 TILES = (32, 64, 128)
 MARGIN = 0.8
 
