@@ -135,6 +135,7 @@ struct WalkArgs {
   unsigned long long* totals;  // [2]
 };
 
+// ---- bh_sort.hip
 // Up to kMax device ranges of 32-bit words to zero (unused: n = 0).
 struct ZeroRanges {
   static constexpr int kMax = 4;
@@ -143,7 +144,6 @@ struct ZeroRanges {
 };
 hipError_t launch_zero(const ZeroRanges& r, hipStream_t s);
 
-// ---- bh_sort.hip
 // chunks[c] = {system, begin, end, 0}: blocks of <= kChunk particles of one
 // system; chunk_first[s] = first chunk of system s (chunk_first[S] = total).
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
