@@ -27,13 +27,15 @@ def device_run(ctx, p, fp):
     return buf[cur].download(np.empty(n, layout.PARTICLE))
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_resident_steps_equal_device_steps(graph):
-    p = ic.plummer(30000, 81)
+@pytest.mark.parametrize("graph,prec,n", [(False, bl.BH_FP32, 30000), (True, bl.BH_FP32, 30000),
+                                          (False, bl.BH_FP64, 30000), (False, bl.BH_FP32, 37)])
+def test_resident_steps_equal_device_steps(graph, prec, n):
+    p = ic.plummer(n, 81)
     fp = layout.force_params(0.5, 1e-4, 1.0, True)
     with bl.Context(0) as ctx:
         ctx.set_graph_mode(bl.BH_GRAPH_ON if graph else bl.BH_GRAPH_OFF)
-        print(f"carried positions graph={graph}", flush=True)
+        ctx.set_precision(prec)
+        print(f"carried positions graph={graph} precision={prec} n={n}", flush=True)
         want = device_run(ctx, p, fp)
         ctx.state_upload(p)
         for kick, drift in SCHED:
