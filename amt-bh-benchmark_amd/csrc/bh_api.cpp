@@ -144,7 +144,7 @@ struct Ctx {
   uint64_t state_n = 0;
   // {x, y, z, m} of the last resident step's output and its non-finite flag,
   // valid (carried_ok) until anything else runs or the state is exposed
-  DevBuf<double4> carry;
+  DevBuf<double> carry;
   DevBuf<uint32_t> carry_flag;
   bool carried_ok = false;
   const void* carried_out = nullptr;
@@ -463,7 +463,7 @@ int provision(Ctx* c, const Plan& p, Scratch* b) {
   TRY(ensure(c, c->leaf_node, m, "leaves"));
   TRY(ensure(c, c->leaf_first, m, "leaves"));
   if (p.carry_out || p.carried_in) {
-    TRY(ensure(c, c->carry, n, "carried positions"));
+    TRY(ensure(c, c->carry, 4ull * n, "carried positions"));
     TRY(ensure(c, c->carry_flag, 1, "carried positions"));
   }
   if (p.walk) {
@@ -644,7 +644,7 @@ int enqueue_build(Ctx* c, const Plan& p, const Scratch& b, const bh_particle* d_
     HIP_OK(c, hipMemcpyAsync(b.hist + 8 * 256, c->host_tables + p.off_hist, 2 * 256 * 4, hipMemcpyHostToDevice, s));
   HIP_OK(c, launch_fill_sid(chunks, p.n_chunks, b.sid, s));
   if (p.carried_in) {
-    HIP_OK(c, launch_bounds(c->carry.p, c->carry_flag.p, chunks, p.n_chunks, chunk_first, p.S, b.partial,
+    HIP_OK(c, launch_bounds(c->carry.p, n, c->carry_flag.p, chunks, p.n_chunks, chunk_first, p.S, b.partial,
                             b.misc + kMiscNonfinite, b.lohi, s));
     HIP_OK(c, launch_keys_hist(c->carry.p, n, b.sid, b.lohi, b.keys_a, b.hist, s));
   } else {

@@ -127,7 +127,7 @@ struct WalkArgs {
   bh_particle* parts;      // Morton-order particles to kick/drift (nullable)
   const bh_particle* src;  // with parts: the unsorted input, read at perm (else parts is read in place)
   const uint32_t* perm;    // [n] Morton position -> input index (with src)
-  double4* carry;          // with parts: [n] the new {x, y, z, m} for the next step's build (nullable)
+  double* carry;           // with parts: [4 n] the new x, y, z and m (blocks of n) for the next step's build (nullable)
   uint32_t* carry_bad;     // with carry: set when a new position, velocity or mass is not finite
   double* acc;             // [3n] G * acceleration (nullable)
   uint32_t* n_pp;          // [n] (nullable)
@@ -152,14 +152,14 @@ hipError_t launch_bounds(const bh_particle* p, const uint4* chunks, uint32_t n_c
                          uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s);
 hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
                             uint32_t* hist, hipStream_t s);
-// The same from the compact positions {x, y, z, m} a resident step's walk
-// wrote for its output (the next step's input): half the bytes of the
-// particle records. carried_bad: that walk's non-finite flag, added to
-// nonfinite.
-hipError_t launch_bounds(const double4* pos, const uint32_t* carried_bad, const uint4* chunks, uint32_t n_chunks,
-                         const uint32_t* chunk_first, uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi,
-                         hipStream_t s);
-hipError_t launch_keys_hist(const double4* pos, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
+// The same from the compact copy a resident step's walk wrote for its
+// output (the next step's input): blocks of n x, y, z and m, of which these
+// read x, y, z (24 of the records' 64 bytes). carried_bad: that walk's
+// non-finite flag, added to nonfinite.
+hipError_t launch_bounds(const double* pos, uint32_t n, const uint32_t* carried_bad, const uint4* chunks,
+                         uint32_t n_chunks, const uint32_t* chunk_first, uint32_t n_sys, double* partial,
+                         uint32_t* nonfinite, double* lohi, hipStream_t s);
+hipError_t launch_keys_hist(const double* pos, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
                             uint32_t* hist, hipStream_t s);
 // 8 passes over the 63-bit keys, then sid_passes (0-2) over the system id
 // taken from sid[index]: a stable sort of each system's (key, index) pairs
@@ -172,9 +172,9 @@ hipError_t launch_radix_sort(uint64_t* keys_a, uint32_t* vals_a, uint64_t* keys_
 // Morton-ordered particles (a step's walk writes those itself).
 hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t* perm, double4* posm, uint32_t n,
                          hipStream_t s);
-// The same walk copy from a carried compact copy (x, y, z, m) in the order
-// perm indexes.
-hipError_t launch_gather(const double4* pos, const uint32_t* perm, double4* posm, uint32_t n, hipStream_t s);
+// The same walk copy from a carried compact copy (blocks of n x, y, z and
+// m) in the order perm indexes.
+hipError_t launch_gather(const double* pos, const uint32_t* perm, double4* posm, uint32_t n, hipStream_t s);
 uint64_t radix_lookback_words(uint32_t n);  // 10 passes of look-back words + 10 tile counters
 
 // ---- bh_tree.hip

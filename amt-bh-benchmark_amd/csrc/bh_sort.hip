@@ -92,28 +92,38 @@ __global__ __launch_bounds__(kBlock) void k_fill_sid(const uint4* __restrict__ c
   for (uint32_t i = c.y + threadIdx.x; i < c.z; i += kBlock) sid[i] = static_cast<uint16_t>(c.x);
 }
 
-// A particle's position for the build, from its record or from the compact
-// {x, y, z, m} copy; `finite` checks what the source holds (the compact
-// copy's velocities were checked by the walk that wrote it).
-__device__ inline void pos_of(const bh_particle& q, double* x, double* y, double* z, bool* finite) {
+// Where the build reads a particle's position: its record, or the compact
+// copy a resident step's walk wrote for its output, four blocks of n
+// doubles (x, y, z, m), of which the build reads x, y, z. `finite` checks
+// what the source holds (the walk that wrote the compact copy flagged any
+// non-finite position, velocity or mass itself).
+struct RecordSrc {
+  const bh_particle* p;
+};
+struct CompactSrc {
+  const double* p;
+  uint64_t n;
+};
+__device__ inline void pos_of(const RecordSrc& s, uint32_t i, double* x, double* y, double* z, bool* finite) {
+  const bh_particle& q = s.p[i];
   *x = q.pos[0];
   *y = q.pos[1];
   *z = q.pos[2];
   *finite = isfinite(*x) && isfinite(*y) && isfinite(*z) && isfinite(q.mass) && isfinite(q.vel[0]) &&
             isfinite(q.vel[1]) && isfinite(q.vel[2]);
 }
-__device__ inline void pos_of(const double4& q, double* x, double* y, double* z, bool* finite) {
-  *x = q.x;
-  *y = q.y;
-  *z = q.z;
-  *finite = isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w);
+__device__ inline void pos_of(const CompactSrc& s, uint32_t i, double* x, double* y, double* z, bool* finite) {
+  *x = s.p[i];
+  *y = s.p[s.n + i];
+  *z = s.p[2 * s.n + i];
+  *finite = isfinite(*x) && isfinite(*y) && isfinite(*z);
 }
 
 // Min/max of the positions of one chunk (flat_build_ttg.h:93-113; min/max
 // is exact, so the chunking does not matter); flags any non-finite
 // position, velocity or mass (and, from a compact copy, its writer's flag).
 template <typename Src>
-__global__ __launch_bounds__(kBlock) void k_bounds_partial(const Src* __restrict__ p, const uint32_t* __restrict__ carried_bad,
+__global__ __launch_bounds__(kBlock) void k_bounds_partial(Src src, const uint32_t* __restrict__ carried_bad,
                                                            const uint4* __restrict__ chunks, double* __restrict__ partial,
                                                            uint32_t* __restrict__ nonfinite) {
   const double inf = __builtin_huge_val();
@@ -130,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_partial(const Src* __restrict
 #pragma unroll
     for (int r = 0; r < kB; ++r) {
       const uint32_t i = i0 + r * kBlock < c.z ? i0 + r * kBlock : c.z - 1;
-      pos_of(p[i], &x[r], &y[r], &z[r], &fin[r]);
+      pos_of(src, i, &x[r], &y[r], &z[r], &fin[r]);
     }
 #pragma unroll
     for (int r = 0; r < kB; ++r) {
@@ -195,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_final(const double* __restric
 // Morton key per particle (input order) and the 8 digit histograms of all
 // keys, for the onesweep passes' global digit offsets.
 template <typename Src>
-__global__ __launch_bounds__(kBlock) void k_keys_hist(const Src* __restrict__ p, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_keys_hist(Src src, uint32_t n,
                                                       const uint16_t* __restrict__ sid, const double* __restrict__ lohi,
                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[8 * 256];
@@ -211,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_keys_hist(const Src* __restrict__ p,
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
       bool fin;
-      pos_of(p[i], &nx, &ny, &nz, &fin);
+      pos_of(src, i, &nx, &ny, &nz, &fin);
       ns = sid[i];
     }
   }
@@ -224,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_keys_hist(const Src* __restrict__ p,
     const uint32_t sys_i = ns;
     if (i + stride < n) {
       bool fin;
-      pos_of(p[i + stride], &nx, &ny, &nz, &fin);
+      pos_of(src, i + stride, &nx, &ny, &nz, &fin);
       ns = sid[i + stride];
     }
     uint64_t key = 0;
@@ -438,17 +448,20 @@ __global__ __launch_bounds__(kBlock) void k_gather(const bh_particle* __restrict
   if (part == 2) q[1] = make_uint4(v.x, v.y, mlo, mhi);
 }
 
-// The walk copy from a carried compact copy (x, y, z, m in the last resident
-// step's order, which is the order perm indexes): two lanes per particle, one
-// 16-byte half each. The copy holds the same FP64 values as the records, so
-// the result equals k_gather's.
-__global__ __launch_bounds__(kBlock) void k_gather_pos(const double4* __restrict__ pos, const uint32_t* __restrict__ perm,
+// The walk copy from a carried compact copy (blocks of n x, y, z and m in
+// the last resident step's order, which is the order perm indexes). The
+// copy holds the same FP64 values as the records, so the result equals
+// k_gather's.
+// Two lanes per particle: lane half 0 moves x, y, half 1 z, m (one 16-byte
+// half of the walk record each).
+__global__ __launch_bounds__(kBlock) void k_gather_pos(const double* __restrict__ pos, const uint32_t* __restrict__ perm,
                                                        double4* __restrict__ posm, uint32_t n) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t k = t >> 1, half = t & 1u;
   if (k >= n) return;
-  const uint32_t j = perm[k];
-  reinterpret_cast<uint4*>(posm + k)[half] = reinterpret_cast<const uint4*>(pos + j)[half];
+  const uint64_t j = perm[k];
+  const uint64_t b0 = (2ull * half) * n + j;
+  reinterpret_cast<double2*>(posm + k)[half] = make_double2(pos[b0], pos[b0 + n]);
 }
 
 inline uint32_t grid_for(uint64_t n, uint32_t cap) {
@@ -481,30 +494,32 @@ hipError_t launch_fill_sid(const uint4* chunks, uint32_t n_chunks, uint16_t* sid
 
 hipError_t launch_bounds(const bh_particle* p, const uint4* chunks, uint32_t n_chunks, const uint32_t* chunk_first,
                          uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi, hipStream_t s) {
-  hipLaunchKernelGGL(k_bounds_partial<bh_particle>, dim3(n_chunks), dim3(kBlock), 0, s, p, nullptr, chunks, partial,
-                     nonfinite);
+  hipLaunchKernelGGL(k_bounds_partial<RecordSrc>, dim3(n_chunks), dim3(kBlock), 0, s, RecordSrc{p}, nullptr, chunks,
+                     partial, nonfinite);
   hipLaunchKernelGGL(k_bounds_final, dim3(n_sys), dim3(kBlock), 0, s, partial, chunk_first, lohi);
   return hipGetLastError();
 }
 
-hipError_t launch_bounds(const double4* pos, const uint32_t* carried_bad, const uint4* chunks, uint32_t n_chunks,
-                         const uint32_t* chunk_first, uint32_t n_sys, double* partial, uint32_t* nonfinite, double* lohi,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_bounds_partial<double4>, dim3(n_chunks), dim3(kBlock), 0, s, pos, carried_bad, chunks, partial,
-                     nonfinite);
+hipError_t launch_bounds(const double* pos, uint32_t n, const uint32_t* carried_bad, const uint4* chunks,
+                         uint32_t n_chunks, const uint32_t* chunk_first, uint32_t n_sys, double* partial,
+                         uint32_t* nonfinite, double* lohi, hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds_partial<CompactSrc>, dim3(n_chunks), dim3(kBlock), 0, s, CompactSrc{pos, n}, carried_bad,
+                     chunks, partial, nonfinite);
   hipLaunchKernelGGL(k_bounds_final, dim3(n_sys), dim3(kBlock), 0, s, partial, chunk_first, lohi);
   return hipGetLastError();
 }
 
 hipError_t launch_keys_hist(const bh_particle* p, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
                             uint32_t* hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_keys_hist<bh_particle>, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, p, n, sid, lohi, keys, hist);
+  hipLaunchKernelGGL(k_keys_hist<RecordSrc>, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, RecordSrc{p}, n, sid, lohi,
+                     keys, hist);
   return hipGetLastError();
 }
 
-hipError_t launch_keys_hist(const double4* pos, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
+hipError_t launch_keys_hist(const double* pos, uint32_t n, const uint16_t* sid, const double* lohi, uint64_t* keys,
                             uint32_t* hist, hipStream_t s) {
-  hipLaunchKernelGGL(k_keys_hist<double4>, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, pos, n, sid, lohi, keys, hist);
+  hipLaunchKernelGGL(k_keys_hist<CompactSrc>, dim3(grid_for(n, 2048)), dim3(kBlock), 0, s, CompactSrc{pos, n}, n, sid,
+                     lohi, keys, hist);
   return hipGetLastError();
 }
 
@@ -554,7 +569,7 @@ hipError_t launch_gather(const bh_particle* in, bh_particle* out, const uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const double4* pos, const uint32_t* perm, double4* posm, uint32_t n, hipStream_t s) {
+hipError_t launch_gather(const double* pos, const uint32_t* perm, double4* posm, uint32_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_pos, dim3((2ull * n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pos, perm, posm, n);
   return hipGetLastError();
 }

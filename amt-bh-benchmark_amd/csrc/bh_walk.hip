@@ -850,8 +850,13 @@ __device__ inline void finish_particle(const WalkArgs& a, uint32_t k3, double ax
     }
     a.parts[k3] = pt;
     if (a.carry) {
-      // the next step's build reads these instead of the records
-      a.carry[k3] = make_double4(pt.pos[0], pt.pos[1], pt.pos[2], pt.mass);
+      // the next step's build reads these instead of the records (x, y, z
+      // and m in blocks of n_parts)
+      const uint64_t np = a.n_parts;
+      a.carry[k3] = pt.pos[0];
+      a.carry[np + k3] = pt.pos[1];
+      a.carry[2 * np + k3] = pt.pos[2];
+      a.carry[3 * np + k3] = pt.mass;
       if (!(isfinite(pt.pos[0]) && isfinite(pt.pos[1]) && isfinite(pt.pos[2]) && isfinite(vx) && isfinite(vy) &&
             isfinite(vz) && isfinite(pt.mass)))
         atomicOr(a.carry_bad, 1u);
